@@ -1,0 +1,255 @@
+#!/usr/bin/env python3
+"""Benchmark of the skinny SGEMM/SGEMV path on one MI355X.
+
+Workload (one "step" = one pass over the plain-FP32 shape sweep, inputs
+resident in HBM):
+  SGEMV y = A x and y = A^T x   for M = N = 16384 and M = 1,048,576, N = 64
+  SGEMM C = A B (row-major)     for M = K = 8192, N in {1,2,4,8,16,32,64,128}
+All entries uniform[-1,1] from fixed seeds, alpha = 1, beta = 0.  Every call
+reads a different copy of its big operand than the call before it, so no
+call re-reads data still held in the 256 MB MALL (cold-HBM numbers).
+
+value = total algorithmic bytes of a step / step time (GB/s), with
+bytes = 4(MK + KN + MN) per SGEMM and 4(MN + M + N) per SGEMV (each operand
+once).  The roofline object reports the kernel that moves the most bytes
+per step (SGEMV 16384 x 16384, A x) timed with HIP events around its
+launches inside the timed steps; cpu_baseline times the restated reference
+CPU path (single thread) on a row sample of every shape.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+SWEEP_GEMV = [(16384, 16384, False), (16384, 16384, True),
+              (1 << 20, 64, False), (1 << 20, 64, True)]
+SWEEP_GEMM = [(8192, n, 8192) for n in (1, 2, 4, 8, 16, 32, 64, 128)]
+
+
+def gemv_bytes(M, N):
+    return 4.0 * (M * N + M + N)
+
+
+def gemm_bytes(M, N, K):
+    return 4.0 * (M * K + K * N + M * N)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="approximate CPU time budget of the baseline sample")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU "
+              "path (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, here)
+    import hiperml_amd as hml
+
+    hml.check(hml.lib.hmlInit())
+    stream = hml.Stream()
+    S = stream.ptr
+
+    # ---- inputs: two copies of every big operand, alternated per call ----
+    bufs = {}
+
+    def big(name, n, seed):
+        if name not in bufs:
+            b = hml.DeviceBuffer(4 * n)
+            b.fill_uniform(n, seed, stream=S)
+            bufs[name] = b
+        return bufs[name]
+
+    calls = []  # (label, kind, shape, fn, bytes, flops)
+    copy = 0
+    for (M, N, trans) in SWEEP_GEMV:
+        A = big("gemvA_%d_%d_%d" % (M, N, copy % 2), M * N, 11 + copy % 2 + M % 7)
+        copy += 1
+        nx, ny = (M, N) if trans else (N, M)
+        x = big("x_%d" % nx, nx, 21)
+        y = hml.DeviceBuffer(4 * ny)
+        fn = (lambda M=M, N=N, A=A, x=x, y=y, trans=trans:
+              hml.sgemv_ex(M, N, A, x, y, 1.0, 0.0, trans=trans, stream=S))
+        calls.append(("sgemv_%s_%dx%d" % ("T" if trans else "N", M, N), "sgemv",
+                      (M, N, trans), fn, gemv_bytes(M, N), 2.0 * M * N, (A, x, y)))
+    for i, (M, N, K) in enumerate(SWEEP_GEMM):
+        A = big("gemmA_%d" % (i % 2), M * K, 31 + i % 2)
+        B = big("B_%d_%d" % (K, N), K * N, 41 + N)
+        C = hml.DeviceBuffer(4 * M * N)
+        ws = hml.lib.hmlSgemmWorkspaceSize(0, 0, 0, M, N, K, None)
+        W = hml.DeviceBuffer(max(ws, 256))
+        fn = (lambda M=M, N=N, K=K, A=A, B=B, C=C, W=W, ws=ws:
+              hml.sgemm_ex(M, N, K, A, B, C, 1.0, 0.0, workspace=W,
+                           ws_bytes=max(ws, 256), stream=S))
+        calls.append(("sgemm_%dx%dx%d" % (M, N, K), "sgemm", (M, N, K), fn,
+                      gemm_bytes(M, N, K), 2.0 * M * N * K, (A, B, C)))
+    stream.synchronize()
+
+    dominant = 0  # SGEMV 16384^2 A x: the most bytes per launch
+    step_bytes = sum(c[4] for c in calls)
+    step_flops = sum(c[5] for c in calls)
+
+    # per-call events (recorded on the launch stream) inside the timed steps
+    ev = [[hml.Event() for _ in range(len(calls) + 1)] for _ in range(args.steps)]
+    for _ in range(args.warmup):
+        for c in calls:
+            c[3]()
+    stream.synchronize()
+    hml.synchronize()
+
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ev[s][0].record(S)
+        for i, c in enumerate(calls):
+            c[3]()
+            ev[s][i + 1].record(S)
+    stream.synchronize()
+    hml.synchronize()
+    wall = time.perf_counter() - t0
+
+    per_call = np.zeros(len(calls))
+    for s in range(args.steps):
+        for i in range(len(calls)):
+            per_call[i] += ev[s][i].elapsed_ms(ev[s][i + 1])
+    per_call /= args.steps
+    gpu_step_ms = ev[0][0].elapsed_ms(ev[-1][-1]) / args.steps
+    ms_per_step = max(gpu_step_ms, 1e-9)
+
+    # accuracy of the last outputs against the FP64 check kernel
+    shapes = []
+    for i, c in enumerate(calls):
+        label, kind, shp, _, nbytes, flops, ops = c
+        if kind == "sgemv":
+            M, N, trans = shp
+            nx, ny = (M, N) if trans else (N, M)
+            st = hml.check_f64(ny, 1, nx, ops[0], ops[1], ops[2], transA=trans,
+                               lda=N, ldb=1, ldc=1)
+            Keff = nx
+        else:
+            M, N, K = shp
+            st = hml.check_f64(M, N, K, ops[0], ops[1], ops[2])
+            Keff = K
+        ms = per_call[i]
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        gfl = flops / (ms * 1e-3) / 1e9
+        ai = flops / nbytes
+        roof_gbs = min(HBM_PEAK_GBS, FP32_PEAK_TFLOPS * 1e3 / ai)
+        shapes.append({"call": label, "ms": round(ms, 5), "GB_s": round(gbs, 1),
+                       "GFLOP_s": round(gfl, 1),
+                       "bound": "hbm" if roof_gbs >= HBM_PEAK_GBS else "fp32",
+                       "frac_of_roofline": round(gbs / roof_gbs, 4),
+                       "rel_fro_err_fp64": st["rel_fro"],
+                       "gammaK_bound": hml.gamma(Keff) * st["bound_fro"]})
+
+    d = calls[dominant]
+    d_ms = per_call[dominant]
+    achieved = d[4] / (d_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "kernel": "k_gemv_dot (SGEMV A x, 16384 x 16384)",
+                "bytes_per_launch": d[4], "avg_launch_ms": round(d_ms, 5)}
+    traffic_file = os.path.join(here, "profiles", "traffic_gemv_dot_16384.json")
+    if os.path.exists(traffic_file):
+        try:
+            roofline["traffic"] = json.load(open(traffic_file)).get("traffic_bytes_per_launch")
+        except Exception:
+            pass
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(hml, calls, args.cpu_seconds)
+
+    out = {
+        "metric": "SGEMM/SGEMV GFLOP/s + HBM GB/s vs plain FP32 at matched FP64-oracle error",
+        "value": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic uniform[-1,1], fixed seeds, inputs resident in HBM",
+        "config": {"workload": "plain FP32 shape sweep: SGEMV Ax/ATx 16384x16384 "
+                               "and 1048576x64; SGEMM 8192x8192xN N=1..128 (row-major)",
+                   "calls_per_step": len(calls), "alpha": 1.0, "beta": 0.0},
+        "gflops": round(step_flops / (ms_per_step * 1e-3) / 1e9, 1),
+        "wall_s_timed": round(wall, 4),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "shapes": shapes,
+    }
+    print(json.dumps(out))
+
+
+def cpu_baseline(hml, calls, budget_s):
+    """Restated reference CPU path (oracle, 1 thread) on a row sample of
+    every call; the full step time is extrapolated per shape from the
+    sample's rows/second and value = step bytes / extrapolated time."""
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle"))
+    import oracle
+    per_shape_budget = budget_s / len(calls)
+    total_s = 0.0
+    total_bytes = 0.0
+    desc = []
+    for label, kind, shp, _, nbytes, flops, ops in calls:
+        if kind == "sgemv":
+            M, N, trans = shp
+            # reference CPU path exists for A x only; A^T x is timed as the
+            # same loop over the transposed rows (same bytes and flops)
+            rows_total, rowlen = (N, M) if trans else (M, N)
+            est_rate = 1.0e9  # flops/s guess to size the first sample
+            r = int(max(1, min(rows_total, per_shape_budget * est_rate / (2.0 * rowlen))))
+            r = min(r, max(1, int(2.0e8 // (rowlen * 4))))  # <= 200 MB sample
+            if trans:
+                Ahost = ops[0].to_numpy((M, N))[:, :r].T.copy()
+            else:
+                Ahost = ops[0].to_numpy((r, N))
+            x = ops[1].to_numpy((rowlen,))
+            y = np.zeros(r, np.float32)
+            t0 = time.perf_counter()
+            oracle.sgemv_cpu(y, Ahost, x, 1.0, 0.0)
+            dt = time.perf_counter() - t0
+            full = dt * rows_total / r
+            desc.append("%s: %d of %d rows" % (label, r, rows_total))
+        else:
+            M, N, K = shp
+            r = int(max(1, min(M, per_shape_budget * 1.0e9 / (2.0 * N * K))))
+            Ahost = ops[0].to_numpy((r, K))
+            Bhost = ops[1].to_numpy((K, N))
+            C = np.zeros((r, N), np.float32)
+            t0 = time.perf_counter()
+            oracle.sgemm_cpu(C, Ahost, Bhost, 1.0, 0.0)
+            dt = time.perf_counter() - t0
+            full = dt * M / r
+            desc.append("%s: %d of %d rows" % (label, r, M))
+        total_s += full
+        total_bytes += nbytes
+    return {"value": round(total_bytes / total_s / 1e9, 3), "unit": "GB/s",
+            "cores": 1, "kind": "port",
+            "sample": "restated hmlSgemmCpu/hmlSgemvCpu (oracle/hml_oracle.c, "
+                      "-O2, 1 thread) on row samples, extrapolated per shape: " +
+                      "; ".join(desc),
+            "step_seconds_extrapolated": round(total_s, 2),
+            "host_cpus": os.cpu_count()}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,514 @@
+// hml_gemm.cpp -- SGEMM dispatch: the reference-compatible hmlSgemm /
+// hmlSgemmBasic (host pointers, exit on error) and the extended
+// device-pointer entry points.
+//
+// Reference flow being replaced (src/hml_sgemm.cu:120-211): lazy repo init,
+// lazy online autotune, cudaMalloc + H2D + texture bind per call, table
+// select (src/hml_sgemm_kernel.cu:189-237), launch, sync, D2H, free.
+// Here the layout (row/column-major, transposes) is first reduced to strides;
+// the larger of A and B becomes the streamed operand so one kernel family
+// serves both tall-skinny and short-wide shapes; then a selector picks the
+// variant and the kernels run on the caller's stream.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "hml_internal.h"
+#include "hml_runtime.h"
+#include "hml_select.h"
+
+namespace hml {
+
+namespace {
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Strides of op(X)(i,j) for a matrix stored in `order` with leading
+// dimension ld: returns (stride of i, stride of j).
+void op_strides(hmlOrder order, hmlOp op, int64_t ld, int64_t *si,
+                int64_t *sj) {
+  const bool rowmajor_view = (order == HML_ROW_MAJOR) == (op == HML_OP_N);
+  if (rowmajor_view) { *si = ld; *sj = 1; } else { *si = 1; *sj = ld; }
+}
+
+hmlStatus check_gemm_args(hmlOrder order, hmlOp ta, hmlOp tb, int64_t M,
+                          int64_t N, int64_t K, const float *A, int64_t lda,
+                          const float *B, int64_t ldb, const float *C,
+                          int64_t ldc) {
+  if (order != HML_ROW_MAJOR && order != HML_COL_MAJOR)
+    return fail(HML_STATUS_INVALID_VALUE, "order must be row- or column-major");
+  if ((ta != HML_OP_N && ta != HML_OP_T) || (tb != HML_OP_N && tb != HML_OP_T))
+    return fail(HML_STATUS_INVALID_VALUE, "trans must be N or T");
+  if (M < 0 || N < 0 || K < 0)
+    return fail(HML_STATUS_INVALID_VALUE, "negative dimension M=%lld N=%lld K=%lld",
+                (long long)M, (long long)N, (long long)K);
+  // BLAS leading-dimension rules: ld >= the extent of the contiguous index.
+  const bool rm = order == HML_ROW_MAJOR;
+  const int64_t a_inner = rm ? (ta == HML_OP_N ? K : M) : (ta == HML_OP_N ? M : K);
+  const int64_t b_inner = rm ? (tb == HML_OP_N ? N : K) : (tb == HML_OP_N ? K : N);
+  const int64_t c_inner = rm ? N : M;
+  if (lda < std::max<int64_t>(1, a_inner))
+    return fail(HML_STATUS_INVALID_VALUE, "lda=%lld too small", (long long)lda);
+  if (ldb < std::max<int64_t>(1, b_inner))
+    return fail(HML_STATUS_INVALID_VALUE, "ldb=%lld too small", (long long)ldb);
+  if (ldc < std::max<int64_t>(1, c_inner))
+    return fail(HML_STATUS_INVALID_VALUE, "ldc=%lld too small", (long long)ldc);
+  if (M > 0 && N > 0 && !C) return fail(HML_STATUS_INVALID_VALUE, "C is NULL");
+  if (M > 0 && N > 0 && K > 0 && (!A || !B))
+    return fail(HML_STATUS_INVALID_VALUE, "A or B is NULL");
+  return HML_STATUS_SUCCESS;
+}
+
+}  // namespace
+
+GemmCanon canonicalize(hmlOrder order, hmlOp ta, hmlOp tb, int64_t M,
+                       int64_t N, int64_t K, const float *A, int64_t lda,
+                       const float *B, int64_t ldb, float *C, int64_t ldc) {
+  GemmCanon g;
+  op_strides(order, ta, lda, &g.sAm, &g.sAk);
+  op_strides(order, tb, ldb, &g.sBk, &g.sBn);
+  if (order == HML_ROW_MAJOR) { g.sCm = ldc; g.sCn = 1; } else { g.sCm = 1; g.sCn = ldc; }
+  g.A = A; g.B = B; g.C = C; g.M = M; g.N = N; g.K = K; g.swapped = false;
+  // Stream the larger operand.  Swapping computes C^T = B^T A^T.
+  if (N > M) {
+    GemmCanon s = g;
+    s.A = g.B; s.sAm = g.sBn; s.sAk = g.sBk;
+    s.B = g.A; s.sBk = g.sAk; s.sBn = g.sAm;
+    s.sCm = g.sCn; s.sCn = g.sCm;
+    s.M = g.N; s.N = g.M;
+    s.swapped = true;
+    g = s;
+  }
+  // Degenerate extents make either stride acceptable.
+  if (g.sAk == 1 || g.K == 1) g.form = kFormDot;
+  else g.form = kFormAxpy;  // sAm == 1 for every BLAS layout here
+  if (g.form == kFormDot && g.K == 1) g.sAk = 1;
+  if (g.form == kFormAxpy && g.M == 1) g.sAm = 1;
+  return g;
+}
+
+GemmKey make_key(const GemmCanon &g, float beta, uint32_t flags, hmlOrder order,
+                 hmlOp ta, hmlOp tb) {
+  GemmKey k;
+  k.form = g.form;
+  k.M = g.M; k.N = g.N; k.K = g.K;
+  k.beta0 = (beta == 0.0f && !(flags & HML_FLAG_READ_C_ALWAYS)) ? 1 : 0;
+  k.transA = ta; k.transB = tb; k.order = order;
+  k.gemv_ok = g.N == 1 && (g.sBk == 1 || g.K == 1) && (g.sCm == 1 || g.M == 1);
+  return k;
+}
+
+// Canonical form of a problem stored densely (leading dimension = extent of
+// the contiguous index), for queries that get no pointers.
+GemmCanon dense_canon(hmlOrder order, hmlOp ta, hmlOp tb, int64_t M,
+                      int64_t N, int64_t K) {
+  const bool rm = order == HML_ROW_MAJOR;
+  const int64_t lda = std::max<int64_t>(1, rm ? (ta == HML_OP_N ? K : M) : (ta == HML_OP_N ? M : K));
+  const int64_t ldb = std::max<int64_t>(1, rm ? (tb == HML_OP_N ? N : K) : (tb == HML_OP_N ? K : N));
+  const int64_t ldc = std::max<int64_t>(1, rm ? N : M);
+  return canonicalize(order, ta, tb, M, N, K, nullptr, lda, nullptr, ldb,
+                      nullptr, ldc);
+}
+
+namespace {
+
+int64_t kstep_of(int form) { return form == kFormDot ? kDotKStep : kAxpyKStep; }
+
+// Resolved launch geometry of a skinny variant.
+struct SkinnyPlan {
+  int nct_pad;
+  int64_t kchunk;
+  int ksplit;
+  size_t bp_floats, p_floats;
+};
+
+SkinnyPlan plan_skinny(const GemmKey &k, const hmlSgemmVariant &v) {
+  SkinnyPlan pl;
+  const int64_t nct = ceil_div(std::max<int64_t>(k.N, 1), 16);
+  pl.nct_pad = (int)(ceil_div(nct, v.ct) * v.ct);
+  const int64_t step = kstep_of(k.form);
+  int64_t ks = std::max<int64_t>(1, v.ksplit);
+  int64_t kchunk = ceil_div(std::max<int64_t>(k.K, 1), ks);
+  kchunk = ceil_div(kchunk, step) * step;
+  pl.kchunk = kchunk;
+  pl.ksplit = (int)ceil_div(std::max<int64_t>(k.K, 1), kchunk);
+  pl.bp_floats = packed_b_floats(k.form, k.K, pl.nct_pad);
+  pl.p_floats = pl.ksplit > 1 ? (size_t)pl.ksplit * k.M * k.N : 0;
+  return pl;
+}
+
+GemvParams gemv_params_from_canon(const GemmCanon &g, int family, float alpha,
+                                  float beta, int read_c) {
+  GemvParams p{};
+  if (family == HML_SGEMV_FAMILY_DOT) {
+    p.A = g.A; p.lda = g.sAm; p.M = g.M; p.N = g.K;
+  } else {
+    p.A = g.A; p.lda = g.sAk; p.M = g.K; p.N = g.M;
+  }
+  p.x = g.B; p.y = g.C;
+  p.alpha = alpha; p.beta = beta; p.read_c = read_c;
+  return p;
+}
+
+}  // namespace
+
+// ---- variant choice -------------------------------------------------------
+
+void heuristic_sgemm(const GemmKey &k, hmlSgemmVariant *v) {
+  if (k.gemv_ok) {
+    GemvKey gk{k.form == kFormDot ? 1 : 2, k.M, k.K, k.beta0};
+    hmlSgemvVariant gv;
+    heuristic_sgemv(gk, &gv);
+    v->family = HML_SGEMM_FAMILY_GEMV;
+    v->rt = gv.lpr; v->ct = gv.rb; v->ksplit = gv.split; v->wpb = gv.wpb;
+    return;
+  }
+  const int64_t nct = ceil_div(k.N, 16);
+  v->ct = nct <= 1 ? 1 : nct <= 2 ? 2 : nct <= 4 ? 4 : 8;
+  v->wpb = 4;
+  int64_t rows_per_wave;
+  if (k.form == kFormDot) {
+    v->family = HML_SGEMM_FAMILY_DOT;
+    v->rt = (v->ct <= 2 && k.M >= 32768) ? 2 : 1;
+    rows_per_wave = 16 * v->rt;
+  } else {
+    v->family = HML_SGEMM_FAMILY_AXPY;
+    v->rt = 1;
+    rows_per_wave = 64;
+  }
+  const int64_t rowblocks = ceil_div(k.M, rows_per_wave) * (ceil_div(nct, v->ct));
+  const int64_t target = 4096;
+  int64_t ks = rowblocks >= target ? 1 : ceil_div(target, rowblocks);
+  ks = std::min<int64_t>(ks, std::max<int64_t>(1, k.K / 512));
+  ks = std::min<int64_t>(ks, 64);
+  v->ksplit = (int)std::max<int64_t>(1, ks);
+}
+
+void basic_sgemm(const GemmKey &k, hmlSgemmVariant *v) {
+  // Fixed fallback, the counterpart of the reference's varK<6,6>
+  // (src/hml_sgemm_kernel.cu:172-184): one geometry for every shape.
+  v->family = k.form == kFormDot ? HML_SGEMM_FAMILY_DOT : HML_SGEMM_FAMILY_AXPY;
+  v->rt = 1;
+  v->ct = 4;
+  v->ksplit = 1;
+  v->wpb = 4;
+}
+
+bool sgemm_variant_runnable(const GemmKey &k, const hmlSgemmVariant &v) {
+  if (v.wpb != 1 && v.wpb != 2 && v.wpb != 4) return false;
+  if (v.family == HML_SGEMM_FAMILY_GEMV) {
+    if (!k.gemv_ok) return false;
+    GemvKey gk{k.form == kFormDot ? 1 : 2, k.M, k.K, k.beta0};
+    hmlSgemvVariant gv{gk.family, v.rt, v.ct, v.ksplit, v.wpb};
+    return sgemv_variant_runnable(gk, gv);
+  }
+  if (v.ksplit < 1 || v.ksplit > 4096) return false;
+  if (v.family == HML_SGEMM_FAMILY_DOT) return k.form == kFormDot && skinny_has_variant(kFormDot, v.rt, v.ct);
+  if (v.family == HML_SGEMM_FAMILY_AXPY) return k.form == kFormAxpy && skinny_has_variant(kFormAxpy, v.rt, v.ct);
+  return false;
+}
+
+void select_sgemm(const GemmKey &k, hmlSgemmVariant *v, int *source) {
+  if (config_select_sgemm(k, v, source) && sgemm_variant_runnable(k, *v)) return;
+  heuristic_sgemm(k, v);
+  *source = 0;
+}
+
+// ---- execution ---------------------------------------------------------------
+
+namespace {
+
+size_t workspace_bytes(const GemmKey &k, const hmlSgemmVariant &v) {
+  if (v.family == HML_SGEMM_FAMILY_GEMV) {
+    return v.ksplit > 1 ? align256((size_t)v.ksplit * k.M * sizeof(float)) : 0;
+  }
+  const SkinnyPlan pl = plan_skinny(k, v);
+  return align256(pl.bp_floats * sizeof(float)) +
+         align256(pl.p_floats * sizeof(float));
+}
+
+hmlStatus run_gemv_family(const GemmCanon &g, const GemmKey &k,
+                          const hmlSgemmVariant &v, float alpha, float beta,
+                          int read_c, void *ws, hipStream_t st);
+
+hmlStatus run_sgemm(const GemmCanon &g, const GemmKey &k,
+                    const hmlSgemmVariant &v, float alpha, float beta,
+                    int read_c, void *ws, size_t ws_bytes, hipStream_t st) {
+  const size_t need = workspace_bytes(k, v);
+  if (need > 0 && !ws) {
+    hmlStatus s = library_workspace(need, &ws);
+    if (s != HML_STATUS_SUCCESS) return s;
+  } else if (need > ws_bytes) {
+    return fail(HML_STATUS_WORKSPACE_TOO_SMALL, "workspace %zu < %zu bytes",
+                ws_bytes, need);
+  }
+  if (v.family == HML_SGEMM_FAMILY_GEMV)
+    return run_gemv_family(g, k, v, alpha, beta, read_c, ws, st);
+
+  const SkinnyPlan pl = plan_skinny(k, v);
+  float *Bp = (float *)ws;
+  float *P = (float *)((char *)ws + align256(pl.bp_floats * sizeof(float)));
+  const int64_t nsteps = ceil_div(k.K, kstep_of(k.form));
+  hipError_t e = launch_pack_b(g.B, g.sBk, g.sBn, g.K, g.N, k.form, nsteps,
+                               pl.nct_pad, Bp, st);
+  if (e != hipSuccess) return from_hip(e, "pack B");
+  SkinnyParams p{};
+  p.A = g.A; p.sAm = g.sAm; p.sAk = g.sAk;
+  p.Bp = Bp;
+  p.C = g.C; p.sCm = g.sCm; p.sCn = g.sCn;
+  p.P = P;
+  p.M = g.M; p.N = g.N; p.K = g.K;
+  p.kchunk = pl.kchunk; p.ksplit = pl.ksplit;
+  p.nct_pad = pl.nct_pad;
+  p.alpha = alpha; p.beta = beta; p.read_c = read_c;
+  bool vec;
+  if (k.form == kFormDot)
+    vec = ((uintptr_t)g.A % 16 == 0) && (g.sAm % 4 == 0 || g.M == 1);
+  else
+    vec = ((uintptr_t)g.A % 16 == 0) && (g.sAk % 4 == 0 || g.K == 1);
+  e = launch_skinny(k.form, v.rt, v.ct, vec, v.wpb, p, st);
+  if (e != hipSuccess) return from_hip(e, "skinny kernel launch");
+  if (pl.ksplit > 1) {
+    e = launch_reduce(P, pl.ksplit, g.M * g.N, g.N, g.C, g.sCm, g.sCn, alpha,
+                      beta, read_c, st);
+    if (e != hipSuccess) return from_hip(e, "split-K reduce");
+  }
+  return HML_STATUS_SUCCESS;
+}
+
+hmlStatus run_gemv_family(const GemmCanon &g, const GemmKey &k,
+                          const hmlSgemmVariant &v, float alpha, float beta,
+                          int read_c, void *ws, hipStream_t st) {
+  const int family = k.form == kFormDot ? HML_SGEMV_FAMILY_DOT : HML_SGEMV_FAMILY_AXPY;
+  GemvParams p = gemv_params_from_canon(g, family, alpha, beta, read_c);
+  p.P = (float *)ws;
+  p.split = std::max(1, v.ksplit);
+  const int64_t reduced = family == HML_SGEMV_FAMILY_DOT ? p.N : p.M;
+  const int64_t outputs = family == HML_SGEMV_FAMILY_DOT ? p.M : p.N;
+  const int64_t gran = family == HML_SGEMV_FAMILY_DOT ? 4 * v.rt : 1;
+  int64_t chunk = ceil_div(std::max<int64_t>(reduced, 1), p.split);
+  chunk = ceil_div(chunk, gran) * gran;
+  p.chunk = chunk;
+  p.split = (int)ceil_div(std::max<int64_t>(reduced, 1), chunk);
+  bool vec = ((uintptr_t)p.A % 16 == 0) && (p.lda % 4 == 0) &&
+             (family != HML_SGEMV_FAMILY_DOT || (uintptr_t)p.x % 16 == 0);
+  hipError_t e = launch_gemv(family, v.rt, v.ct, vec, v.wpb, p, st);
+  if (e != hipSuccess) return from_hip(e, "gemv kernel launch");
+  if (p.split > 1) {
+    e = launch_reduce(p.P, p.split, outputs, 1, p.y, 1, 0, alpha, beta,
+                      read_c, st);
+    if (e != hipSuccess) return from_hip(e, "gemv split reduce");
+  }
+  return HML_STATUS_SUCCESS;
+}
+
+hmlStatus sgemm_impl(hmlOrder order, hmlOp ta, hmlOp tb, int64_t M, int64_t N,
+                     int64_t K, float alpha, const float *A, int64_t lda,
+                     const float *B, int64_t ldb, float beta, float *C,
+                     int64_t ldc, uint32_t flags, const hmlSgemmVariant *forced,
+                     void *ws, size_t ws_bytes, hipStream_t st,
+                     hmlSgemmVariant *used) {
+  clear_error();
+  hmlStatus s = check_gemm_args(order, ta, tb, M, N, K, A, lda, B, ldb, C, ldc);
+  if (s != HML_STATUS_SUCCESS) return s;
+  if (M == 0 || N == 0) return HML_STATUS_SUCCESS;
+  const int read_c = (beta != 0.0f || (flags & HML_FLAG_READ_C_ALWAYS)) ? 1 : 0;
+  GemmCanon g = canonicalize(order, ta, tb, M, N, K, A, lda, B, ldb, C, ldc);
+  if (K == 0 || (alpha == 0.0f && !(flags & HML_FLAG_READ_C_ALWAYS))) {
+    // No product term: C = alpha*0 + beta*C (BLAS quick return; with the
+    // reference semantics alpha == 0 still runs the product so that
+    // 0 * Inf gives NaN exactly as the reference kernels do).
+    if (read_c && beta == 1.0f) return HML_STATUS_SUCCESS;
+    return from_hip(launch_scale(g.C, g.M, g.N, g.sCm, g.sCn, alpha, beta,
+                                 read_c, st),
+                    "scale C");
+  }
+  GemmKey key = make_key(g, beta, flags, order, ta, tb);
+  hmlSgemmVariant v;
+  if (forced) {
+    v = *forced;
+    if (!sgemm_variant_runnable(key, v))
+      return fail(HML_STATUS_NO_KERNEL,
+                  "variant family=%d rt=%d ct=%d ksplit=%d wpb=%d cannot run form %d",
+                  v.family, v.rt, v.ct, v.ksplit, v.wpb, key.form);
+  } else {
+    int src;
+    select_sgemm(key, &v, &src);
+  }
+  if (used) *used = v;
+  return run_sgemm(g, key, v, alpha, beta, read_c, ws, ws_bytes, st);
+}
+
+// Host-pointer compatibility path (src/hml_sgemm.cu:39-113,120-211).
+void sgemm_compat(float *C, const float *A, const float *B, int M, int N,
+                  int K, float alpha, float beta, bool basic) {
+  if (M < 0 || N < 0 || K < 0) {
+    fprintf(stderr, "; Error: invalid SGEMM dimensions\n");
+    exit(1);
+  }
+  const size_t szA = (size_t)M * K, szB = (size_t)K * N, szC = (size_t)M * N;
+  float *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  hmlStatus s;
+  if ((s = hmlDeviceMalloc((void **)&dA, szA * 4)) != HML_STATUS_SUCCESS) compat_die("alloc A", s);
+  if ((s = hmlDeviceMalloc((void **)&dB, szB * 4)) != HML_STATUS_SUCCESS) compat_die("alloc B", s);
+  if ((s = hmlDeviceMalloc((void **)&dC, szC * 4)) != HML_STATUS_SUCCESS) compat_die("alloc C", s);
+  if ((s = hmlMemcpyHtoD(dA, A, szA * 4)) != HML_STATUS_SUCCESS) compat_die("copy A", s);
+  if ((s = hmlMemcpyHtoD(dB, B, szB * 4)) != HML_STATUS_SUCCESS) compat_die("copy B", s);
+  if ((s = hmlMemcpyHtoD(dC, C, szC * 4)) != HML_STATUS_SUCCESS) compat_die("copy C", s);
+  hmlSgemmVariant v;
+  GemmCanon g = canonicalize(HML_ROW_MAJOR, HML_OP_N, HML_OP_N, M, N, K, dA,
+                             std::max(K, 1), dB, std::max(N, 1), dC, std::max(N, 1));
+  GemmKey key = make_key(g, beta, HML_FLAG_READ_C_ALWAYS, HML_ROW_MAJOR,
+                         HML_OP_N, HML_OP_N);
+  int src = 0;
+  if (basic) basic_sgemm(key, &v); else select_sgemm(key, &v, &src);
+  if (verbosity() >= 1 && !basic)
+    fprintf(stderr, "Info: kernel type = %d, colStops = %d, rowStops = %d\n",
+            v.family, v.ct, v.rt);
+  s = sgemm_impl(HML_ROW_MAJOR, HML_OP_N, HML_OP_N, M, N, K, alpha, dA,
+                 std::max(K, 1), dB, std::max(N, 1), beta, dC, std::max(N, 1),
+                 HML_FLAG_READ_C_ALWAYS, &v, nullptr, 0, nullptr, nullptr);
+  if (s != HML_STATUS_SUCCESS) {
+    fprintf(stderr, "Run kernel: %s\n", hmlLastError());
+    exit(1);
+  }
+  if ((s = hmlDeviceSynchronize()) != HML_STATUS_SUCCESS) {
+    fprintf(stderr, "Run kernel: %s\n", hmlLastError());
+    exit(1);
+  }
+  if ((s = hmlMemcpyDtoH(C, dC, szC * 4)) != HML_STATUS_SUCCESS) {
+    fprintf(stderr, "Copy C off of device: %s\n", hmlLastError());
+    exit(1);
+  }
+  hmlDeviceFree(dA);
+  hmlDeviceFree(dB);
+  hmlDeviceFree(dC);
+}
+
+}  // namespace
+}  // namespace hml
+
+using namespace hml;
+
+extern "C" {
+
+hmlStatus hmlSgemmEx(hmlOrder order, hmlOp transA, hmlOp transB, int64_t M,
+                     int64_t N, int64_t K, float alpha, const float *A,
+                     int64_t lda, const float *B, int64_t ldb, float beta,
+                     float *C, int64_t ldc, uint32_t flags, void *workspace,
+                     size_t ws_bytes, void *stream) {
+  return sgemm_impl(order, transA, transB, M, N, K, alpha, A, lda, B, ldb,
+                    beta, C, ldc, flags, nullptr, workspace, ws_bytes,
+                    (hipStream_t)stream, nullptr);
+}
+
+hmlStatus hmlSgemmExVariant(hmlOrder order, hmlOp transA, hmlOp transB,
+                            int64_t M, int64_t N, int64_t K, float alpha,
+                            const float *A, int64_t lda, const float *B,
+                            int64_t ldb, float beta, float *C, int64_t ldc,
+                            uint32_t flags, const hmlSgemmVariant *variant,
+                            void *workspace, size_t ws_bytes, void *stream) {
+  if (!variant) return fail(HML_STATUS_INVALID_VALUE, "variant is NULL");
+  return sgemm_impl(order, transA, transB, M, N, K, alpha, A, lda, B, ldb,
+                    beta, C, ldc, flags, variant, workspace, ws_bytes,
+                    (hipStream_t)stream, nullptr);
+}
+
+size_t hmlSgemmWorkspaceSize(hmlOrder order, hmlOp transA, hmlOp transB,
+                             int64_t M, int64_t N, int64_t K,
+                             const hmlSgemmVariant *variant) {
+  if (M <= 0 || N <= 0 || K <= 0) return 0;
+  GemmCanon g = dense_canon(order, transA, transB, M, N, K);
+  // The strides are unknown here, so when N == 1 after orientation the call
+  // may take either the GEMV or the skinny route: return the larger need.
+  GemmKey key = make_key(g, 0.0f, 0, order, transA, transB);
+  size_t need = 0;
+  for (int gemv = 0; gemv <= (g.N == 1 ? 1 : 0); ++gemv) {
+    key.gemv_ok = gemv == 1;
+    hmlSgemmVariant v;
+    if (variant) {
+      v = *variant;
+    } else {
+      int src;
+      select_sgemm(key, &v, &src);
+    }
+    if (!variant || sgemm_variant_runnable(key, v))
+      need = std::max(need, workspace_bytes(key, v));
+  }
+  return need;
+}
+
+hmlStatus hmlSgemmSelect(hmlOrder order, hmlOp transA, hmlOp transB,
+                         int64_t M, int64_t N, int64_t K, float beta,
+                         hmlSgemmVariant *variant, int32_t *source) {
+  if (!variant) return fail(HML_STATUS_INVALID_VALUE, "variant is NULL");
+  if (M <= 0 || N <= 0 || K <= 0)
+    return fail(HML_STATUS_INVALID_VALUE, "dimensions must be positive");
+  GemmCanon g = dense_canon(order, transA, transB, M, N, K);
+  GemmKey key = make_key(g, beta, 0, order, transA, transB);
+  int src = 0;
+  select_sgemm(key, variant, &src);
+  if (source) *source = src;
+  return HML_STATUS_SUCCESS;
+}
+
+hmlStatus hmlSgemmCheckF64(hmlOrder order, hmlOp transA, hmlOp transB,
+                           int64_t M, int64_t N, int64_t K, float alpha,
+                           const float *A, int64_t lda, const float *B,
+                           int64_t ldb, float beta, const float *C0,
+                           const float *C, int64_t ldc, double *stats) {
+  clear_error();
+  hmlStatus s = check_gemm_args(order, transA, transB, M, N, K, A, lda, B, ldb, C, ldc);
+  if (s != HML_STATUS_SUCCESS) return s;
+  if (!stats) return fail(HML_STATUS_INVALID_VALUE, "stats is NULL");
+  for (int i = 0; i < 6; ++i) stats[i] = 0.0;
+  if (M == 0 || N == 0) return HML_STATUS_SUCCESS;
+  CheckParams p{};
+  op_strides(order, transA, lda, &p.sAm, &p.sAk);
+  op_strides(order, transB, ldb, &p.sBk, &p.sBn);
+  if (order == HML_ROW_MAJOR) { p.sCm = ldc; p.sCn = 1; } else { p.sCm = 1; p.sCn = ldc; }
+  p.A = A; p.B = B; p.C0 = beta != 0.0f ? C0 : nullptr; p.C = C;
+  p.M = M; p.N = N; p.K = K; p.alpha = alpha; p.beta = beta;
+  if (beta != 0.0f && !C0) return fail(HML_STATUS_INVALID_VALUE, "C0 needed when beta != 0");
+  const int64_t nblk = ((M + 15) / 16) * ((N + 15) / 16);
+  double *dstats = nullptr;
+  hipError_t e = hipMalloc(&dstats, (size_t)nblk * 6 * sizeof(double));
+  if (e != hipSuccess) return from_hip(e, "hipMalloc(check stats)");
+  p.stats = dstats;
+  int64_t nb = 0;
+  e = launch_check_f64(p, &nb, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    hipFree(dstats);
+    return from_hip(e, "check kernel");
+  }
+  double *h = (double *)malloc((size_t)nblk * 6 * sizeof(double));
+  if (!h) { hipFree(dstats); return fail(HML_STATUS_ALLOC_FAILED, "host stats"); }
+  e = hipMemcpy(h, dstats, (size_t)nblk * 6 * sizeof(double), hipMemcpyDeviceToHost);
+  hipFree(dstats);
+  if (e != hipSuccess) { free(h); return from_hip(e, "copy check stats"); }
+  for (int64_t b = 0; b < nblk; ++b) {
+    stats[0] += h[b * 6 + 0];
+    stats[1] += h[b * 6 + 1];
+    stats[2] += h[b * 6 + 2];
+    stats[3] = std::max(stats[3], h[b * 6 + 3]);
+    stats[4] = std::max(stats[4], h[b * 6 + 4]);
+    stats[5] += h[b * 6 + 5];
+  }
+  free(h);
+  return HML_STATUS_SUCCESS;
+}
+
+void hmlSgemmBasic(float *C, const float *A, const float *B, const int M,
+                   const int N, const int K, const float alpha,
+                   const float beta) {
+  sgemm_compat(C, A, B, M, N, K, alpha, beta, true);
+}
+
+void hmlSgemm(float *C, const float *A, const float *B, const int M,
+              const int N, const int K, const float alpha, const float beta) {
+  sgemm_compat(C, A, B, M, N, K, alpha, beta, false);
+}
+
+}  // extern "C"

@@ -1,0 +1,297 @@
+// hml_gemv.cpp -- SGEMV dispatch: reference-compatible hmlSgemv /
+// hmlSgemvBasic (host pointers, exit on error) and the extended
+// device-pointer entry points.
+//
+// Reference flow being replaced: src/hml_sgemv.cu:125-213 (lazy init, lazy
+// online autotune, alloc + copy + bind, hmlSgemvKernelSelect
+// src/hml_sgemv_kernel.cu:315-355, launch, sync, copy back).  Every layout is
+// reduced to one of two forms: DOT (each output is a contiguous row of A) or
+// AXPY (each output is a column, rows of A contiguous).
+#include <stdio.h>
+
+#include <algorithm>
+
+#include "hml_internal.h"
+#include "hml_runtime.h"
+#include "hml_select.h"
+
+namespace hml {
+
+namespace {
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+int lpr_for(int64_t len) {
+  // Lanes per row / column chunk: enough lanes that one 128-bit load per
+  // lane covers the row (or column chunk), between 4 and 64.
+  int lpr = 4;
+  while (lpr < 64 && (int64_t)lpr * 4 < len) lpr <<= 1;
+  return lpr;
+}
+
+struct GemvCanon {
+  GemvParams p;   // without split / chunk / partials
+  int family;     // 1 DOT, 2 AXPY
+  int64_t outputs, reduced;
+};
+
+GemvCanon canonicalize(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
+                       const float *A, int64_t lda, const float *x, float *y) {
+  GemvCanon c;
+  GemvParams &p = c.p;
+  p = GemvParams{};
+  p.A = A; p.lda = lda; p.x = x; p.y = y;
+  // Row-major N and column-major T: outputs are contiguous rows of A.
+  const bool dot = (order == HML_ROW_MAJOR) == (trans == HML_OP_N);
+  const int64_t out = trans == HML_OP_N ? M : N;
+  const int64_t red = trans == HML_OP_N ? N : M;
+  if (dot) {
+    c.family = HML_SGEMV_FAMILY_DOT;
+    p.M = out; p.N = red;
+  } else {
+    c.family = HML_SGEMV_FAMILY_AXPY;
+    p.M = red; p.N = out;
+  }
+  c.outputs = out; c.reduced = red;
+  return c;
+}
+
+}  // namespace
+
+GemvKey gemv_key(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
+                 int beta0) {
+  GemvCanon c = canonicalize(order, trans, M, N, nullptr, 1, nullptr, nullptr);
+  return GemvKey{c.family, c.outputs, c.reduced, beta0};
+}
+
+int ilog2_floor(int64_t x) {
+  int l = 0;
+  while (x > 1) { x >>= 1; ++l; }
+  return l;
+}
+
+void heuristic_sgemv(const GemvKey &k, hmlSgemvVariant *v) {
+  v->family = k.family;
+  v->wpb = 4;
+  if (k.family == HML_SGEMV_FAMILY_DOT) {
+    v->lpr = lpr_for(k.reduced);
+    const int G = 64 / v->lpr;
+    v->rb = v->lpr == 64 ? 4 : (G >= 4 ? 4 : 4);
+    const int64_t rows_per_wave = (int64_t)G * v->rb;
+    const int64_t waves = ceil_div(k.outputs, rows_per_wave);
+    int64_t split = 1;
+    if (waves < 2048) {
+      split = ceil_div(2048, waves);
+      split = std::min<int64_t>(split, std::max<int64_t>(1, k.reduced / (4 * v->lpr * 8)));
+    }
+    v->split = (int)std::max<int64_t>(1, std::min<int64_t>(split, 1024));
+  } else {
+    v->lpr = lpr_for(k.outputs);
+    v->rb = 1;
+    const int64_t ncb = ceil_div(k.outputs, 4 * v->lpr);
+    int64_t split = ceil_div(1024, ncb);
+    split = std::min<int64_t>(split, std::max<int64_t>(1, k.reduced / 256));
+    v->split = (int)std::max<int64_t>(1, std::min<int64_t>(split, 4096));
+  }
+}
+
+void basic_sgemv(const GemvKey &k, hmlSgemvVariant *v) {
+  // Fixed fallback, the counterpart of the reference's basic[8]
+  // (src/hml_sgemv_kernel.cu:294-310): whole-wave rows, no split.
+  v->family = k.family;
+  v->lpr = 64;
+  v->rb = k.family == HML_SGEMV_FAMILY_DOT ? 1 : 1;
+  v->split = k.family == HML_SGEMV_FAMILY_DOT ? 1 : (int)std::max<int64_t>(1, std::min<int64_t>(k.reduced / 1024, 1024));
+  v->wpb = 4;
+}
+
+bool sgemv_variant_runnable(const GemvKey &k, const hmlSgemvVariant &v) {
+  if (v.family != k.family) return false;
+  if (v.wpb != 1 && v.wpb != 2 && v.wpb != 4) return false;
+  if (v.split < 1 || v.split > 65535) return false;
+  return gemv_has_variant(v.family, v.lpr, v.rb);
+}
+
+void select_sgemv(const GemvKey &k, hmlSgemvVariant *v, int *source) {
+  if (config_select_sgemv(k, v, source) && sgemv_variant_runnable(k, *v)) return;
+  heuristic_sgemv(k, v);
+  *source = 0;
+}
+
+namespace {
+
+size_t gemv_workspace(const GemvKey &k, const hmlSgemvVariant &v) {
+  return v.split > 1 ? align256((size_t)v.split * k.outputs * sizeof(float)) : 0;
+}
+
+hmlStatus sgemv_impl(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
+                     float alpha, const float *A, int64_t lda, const float *x,
+                     int64_t incx, float beta, float *y, int64_t incy,
+                     uint32_t flags, const hmlSgemvVariant *forced, void *ws,
+                     size_t ws_bytes, hipStream_t st) {
+  clear_error();
+  if (order != HML_ROW_MAJOR && order != HML_COL_MAJOR)
+    return fail(HML_STATUS_INVALID_VALUE, "order must be row- or column-major");
+  if (trans != HML_OP_N && trans != HML_OP_T)
+    return fail(HML_STATUS_INVALID_VALUE, "trans must be N or T");
+  if (M < 0 || N < 0) return fail(HML_STATUS_INVALID_VALUE, "negative dimension");
+  const int64_t inner = order == HML_ROW_MAJOR ? N : M;
+  if (lda < std::max<int64_t>(1, inner))
+    return fail(HML_STATUS_INVALID_VALUE, "lda=%lld too small", (long long)lda);
+  if (incx != 1 || incy != 1)
+    return fail(HML_STATUS_UNSUPPORTED, "only incx == incy == 1 is supported");
+  GemvCanon c = canonicalize(order, trans, M, N, A, lda, x, y);
+  if (c.outputs == 0) return HML_STATUS_SUCCESS;
+  if (!y) return fail(HML_STATUS_INVALID_VALUE, "y is NULL");
+  const int read_c = (beta != 0.0f || (flags & HML_FLAG_READ_C_ALWAYS)) ? 1 : 0;
+  if (c.reduced == 0 || (alpha == 0.0f && !(flags & HML_FLAG_READ_C_ALWAYS))) {
+    if (read_c && beta == 1.0f) return HML_STATUS_SUCCESS;
+    return from_hip(launch_scale(y, c.outputs, 1, 1, 0, alpha, beta, read_c, st),
+                    "scale y");
+  }
+  if (!A || !x) return fail(HML_STATUS_INVALID_VALUE, "A or x is NULL");
+  GemvKey key{c.family, c.outputs, c.reduced,
+              (beta == 0.0f && !(flags & HML_FLAG_READ_C_ALWAYS)) ? 1 : 0};
+  hmlSgemvVariant v;
+  if (forced) {
+    v = *forced;
+    if (!sgemv_variant_runnable(key, v))
+      return fail(HML_STATUS_NO_KERNEL, "gemv variant family=%d lpr=%d rb=%d split=%d wpb=%d cannot run family %d",
+                  v.family, v.lpr, v.rb, v.split, v.wpb, key.family);
+  } else {
+    int src;
+    select_sgemv(key, &v, &src);
+  }
+  GemvParams p = c.p;
+  p.alpha = alpha; p.beta = beta; p.read_c = read_c;
+  const int64_t gran = c.family == HML_SGEMV_FAMILY_DOT ? 4 * v.lpr : 1;
+  int64_t chunk = ceil_div(c.reduced, v.split);
+  chunk = ceil_div(chunk, gran) * gran;
+  p.chunk = chunk;
+  p.split = (int)ceil_div(c.reduced, chunk);
+  const size_t need = p.split > 1 ? align256((size_t)p.split * c.outputs * sizeof(float)) : 0;
+  if (need > 0) {
+    if (!ws) {
+      hmlStatus s = library_workspace(need, &ws);
+      if (s != HML_STATUS_SUCCESS) return s;
+    } else if (ws_bytes < need) {
+      return fail(HML_STATUS_WORKSPACE_TOO_SMALL, "workspace %zu < %zu bytes", ws_bytes, need);
+    }
+  }
+  p.P = (float *)ws;
+  const bool vec = ((uintptr_t)A % 16 == 0) && (lda % 4 == 0) &&
+                   (c.family != HML_SGEMV_FAMILY_DOT || (uintptr_t)x % 16 == 0);
+  hipError_t e = launch_gemv(v.family, v.lpr, v.rb, vec, v.wpb, p, st);
+  if (e != hipSuccess) return from_hip(e, "gemv kernel launch");
+  if (p.split > 1) {
+    e = launch_reduce(p.P, p.split, c.outputs, 1, y, 1, 0, alpha, beta, read_c, st);
+    if (e != hipSuccess) return from_hip(e, "gemv split reduce");
+  }
+  return HML_STATUS_SUCCESS;
+}
+
+void sgemv_compat(float *y, const float *A, const float *x, int M, int N,
+                  float alpha, float beta, bool basic) {
+  if (M < 0 || N < 0) {
+    fprintf(stderr, "; Error: invalid SGEMV dimensions\n");
+    exit(1);
+  }
+  const size_t szA = (size_t)M * N;
+  float *dA = nullptr, *dx = nullptr, *dy = nullptr;
+  hmlStatus s;
+  if ((s = hmlDeviceMalloc((void **)&dA, szA * 4)) != HML_STATUS_SUCCESS) compat_die("alloc A", s);
+  if ((s = hmlDeviceMalloc((void **)&dx, (size_t)N * 4)) != HML_STATUS_SUCCESS) compat_die("alloc x", s);
+  if ((s = hmlDeviceMalloc((void **)&dy, (size_t)M * 4)) != HML_STATUS_SUCCESS) compat_die("alloc y", s);
+  if ((s = hmlMemcpyHtoD(dA, A, szA * 4)) != HML_STATUS_SUCCESS) compat_die("copy A", s);
+  if ((s = hmlMemcpyHtoD(dx, x, (size_t)N * 4)) != HML_STATUS_SUCCESS) compat_die("copy x", s);
+  if ((s = hmlMemcpyHtoD(dy, y, (size_t)M * 4)) != HML_STATUS_SUCCESS) compat_die("copy y", s);
+  GemvKey key{HML_SGEMV_FAMILY_DOT, M, N, 0};
+  hmlSgemvVariant v;
+  int src = 0;
+  if (basic) basic_sgemv(key, &v); else select_sgemv(key, &v, &src);
+  if (verbosity() >= 1 && !basic)
+    fprintf(stderr, "Info: kernel type = %d, blockStops = %d, rowStops = %d\n",
+            v.family, v.lpr / 16 > 0 ? v.lpr / 16 : 1, v.rb);
+  s = sgemv_impl(HML_ROW_MAJOR, HML_OP_N, M, N, alpha, dA, std::max(N, 1), dx,
+                 1, beta, dy, 1, HML_FLAG_READ_C_ALWAYS, &v, nullptr, 0, nullptr);
+  if (s == HML_STATUS_SUCCESS) s = hmlDeviceSynchronize();
+  if (s != HML_STATUS_SUCCESS) {
+    fprintf(stderr, "Run kernel: %s\n", hmlLastError());
+    exit(1);
+  }
+  if ((s = hmlMemcpyDtoH(y, dy, (size_t)M * 4)) != HML_STATUS_SUCCESS) {
+    fprintf(stderr, "Copy y off of device: %s\n", hmlLastError());
+    exit(1);
+  }
+  hmlDeviceFree(dA);
+  hmlDeviceFree(dx);
+  hmlDeviceFree(dy);
+}
+
+}  // namespace
+}  // namespace hml
+
+using namespace hml;
+
+extern "C" {
+
+hmlStatus hmlSgemvEx(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
+                     float alpha, const float *A, int64_t lda, const float *x,
+                     int64_t incx, float beta, float *y, int64_t incy,
+                     uint32_t flags, void *workspace, size_t ws_bytes,
+                     void *stream) {
+  return sgemv_impl(order, trans, M, N, alpha, A, lda, x, incx, beta, y, incy,
+                    flags, nullptr, workspace, ws_bytes, (hipStream_t)stream);
+}
+
+hmlStatus hmlSgemvExVariant(hmlOrder order, hmlOp trans, int64_t M,
+                            int64_t N, float alpha, const float *A,
+                            int64_t lda, const float *x, int64_t incx,
+                            float beta, float *y, int64_t incy,
+                            uint32_t flags, const hmlSgemvVariant *variant,
+                            void *workspace, size_t ws_bytes, void *stream) {
+  if (!variant) return fail(HML_STATUS_INVALID_VALUE, "variant is NULL");
+  return sgemv_impl(order, trans, M, N, alpha, A, lda, x, incx, beta, y, incy,
+                    flags, variant, workspace, ws_bytes, (hipStream_t)stream);
+}
+
+size_t hmlSgemvWorkspaceSize(hmlOrder order, hmlOp trans, int64_t M,
+                             int64_t N, const hmlSgemvVariant *variant) {
+  if (M <= 0 || N <= 0) return 0;
+  GemvCanon c = canonicalize(order, trans, M, N, nullptr, 1, nullptr, nullptr);
+  GemvKey key{c.family, c.outputs, c.reduced, 0};
+  hmlSgemvVariant v;
+  if (variant) {
+    v = *variant;
+  } else {
+    int src;
+    select_sgemv(key, &v, &src);
+  }
+  // The launch may use fewer splits than requested, never more.
+  return gemv_workspace(key, v);
+}
+
+hmlStatus hmlSgemvSelect(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
+                         float beta, hmlSgemvVariant *variant,
+                         int32_t *source) {
+  if (!variant) return fail(HML_STATUS_INVALID_VALUE, "variant is NULL");
+  if (M <= 0 || N <= 0) return fail(HML_STATUS_INVALID_VALUE, "dimensions must be positive");
+  GemvCanon c = canonicalize(order, trans, M, N, nullptr, 1, nullptr, nullptr);
+  GemvKey key{c.family, c.outputs, c.reduced, beta == 0.0f ? 1 : 0};
+  int src = 0;
+  select_sgemv(key, variant, &src);
+  if (source) *source = src;
+  return HML_STATUS_SUCCESS;
+}
+
+void hmlSgemvBasic(float *y, const float *A, const float *x, const int M,
+                   const int N, const float alpha, const float beta) {
+  sgemv_compat(y, A, x, M, N, alpha, beta, true);
+}
+
+void hmlSgemv(float *y, const float *A, const float *x, const int M,
+              const int N, const float alpha, const float beta) {
+  sgemv_compat(y, A, x, M, N, alpha, beta, false);
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+// hml_internal.h -- shared declarations between the host library and the
+// HIP kernel translation units.  Not installed; not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+namespace hml {
+
+// Streamed-operand forms of the skinny kernels (see hml_k_skinny.hip).
+enum Form : int {
+  kFormDot = 0,   // A(m,k) at A[m*sAm + k], contiguous along K
+  kFormAxpy = 1,  // A(m,k) at A[m + k*sAk], contiguous along M
+};
+
+// k-step (K values consumed per loop iteration) of each form.  The packed B
+// layout and the K-split granularity are multiples of it.
+constexpr int kDotKStep = 32;
+constexpr int kAxpyKStep = 16;
+// Rows covered by one "row tile" of each form.
+constexpr int kDotRowsPerTile = 16;
+constexpr int kAxpyRowsPerTile = 64;
+
+struct SkinnyParams {
+  const float *A;   // streamed operand
+  int64_t sAm, sAk; // its strides (one of them is 1)
+  const float *Bp;  // B packed in MFMA fragment order (zero padded)
+  float *C;         // output
+  int64_t sCm, sCn;
+  float *P;         // partials [ksplit][M*N] when ksplit > 1
+  int64_t M, N, K;
+  int64_t kchunk;   // K range per split, multiple of the k-step
+  int32_t ksplit;
+  int32_t nct_pad;  // column tiles held in Bp (multiple of the variant's CT)
+  float alpha, beta;
+  int32_t read_c;   // C participates in the epilogue
+};
+
+struct GemvParams {
+  const float *A;
+  int64_t lda;
+  const float *x;
+  float *y;
+  float *P;         // partials [split][outputs] when split > 1
+  int64_t M, N;     // DOT: M outputs, rows of length N; AXPY: M reduced rows, N outputs
+  int64_t chunk;    // length of the reduced dimension per split
+  int32_t split;
+  float alpha, beta;
+  int32_t read_c;
+};
+
+// Packs B(k,n) = B[k*sBk + n*sBn] into fragment order for `form`.
+hipError_t launch_pack_b(const float *B, int64_t sBk, int64_t sBn, int64_t K,
+                         int64_t N, int form, int64_t nsteps, int nct_pad,
+                         float *Bp, hipStream_t st);
+size_t packed_b_floats(int form, int64_t K, int nct_pad);
+
+// Returns hipErrorInvalidValue when (form, rt, ct) has no instantiation.
+hipError_t launch_skinny(int form, int rt, int ct, bool vec, int wpb,
+                         const SkinnyParams &p, hipStream_t st);
+bool skinny_has_variant(int form, int rt, int ct);
+
+// out(m,n) = alpha * sum_r P[r*total + m*ninner + n] + beta * out(m,n)
+hipError_t launch_reduce(const float *P, int R, int64_t total, int64_t ninner,
+                         float *C, int64_t sCm, int64_t sCn, float alpha,
+                         float beta, int read_c, hipStream_t st);
+
+// Epilogue only (K == 0, or alpha == 0 under BLAS rules):
+// C = alpha*0 + beta*C, or alpha*0 when !read_c.
+hipError_t launch_scale(float *C, int64_t M, int64_t N, int64_t sCm,
+                        int64_t sCn, float alpha, float beta, int read_c,
+                        hipStream_t st);
+
+hipError_t launch_gemv(int family, int lpr, int rb, bool vec, int wpb,
+                       const GemvParams &p, hipStream_t st);
+bool gemv_has_variant(int family, int lpr, int rb);
+
+struct CheckParams {
+  const float *A; int64_t sAm, sAk;
+  const float *B; int64_t sBk, sBn;
+  const float *C0; const float *C; int64_t sCm, sCn;
+  int64_t M, N, K;
+  double alpha, beta;
+  double *stats;    // [gridDim.x*gridDim.y][6] per-block partial statistics
+};
+hipError_t launch_check_f64(const CheckParams &p, int64_t *nblocks,
+                            hipStream_t st);
+
+hipError_t launch_fill(float *dst, size_t n, float value, hipStream_t st);
+
+}  // namespace hml
+
+namespace hml {
+// dst[i] = lo + (hi-lo) * u_i, u_i from a counter-based splitmix64 stream.
+hipError_t launch_fill_uniform(float *dst, size_t n, uint64_t seed, float lo,
+                               float hi, hipStream_t st);
+// Per-block statistics of C vs R: [sum (C-R)^2, sum R^2, max|C-R|, max|R|,
+// count non-finite C where R is finite], nblocks x 5 doubles.
+constexpr int kCompareBlocks = 1024;
+hipError_t launch_compare(const float *C, const float *R, int64_t M, int64_t N,
+                          int64_t sCm, int64_t sCn, double *stats, int nblocks,
+                          hipStream_t st);
+}  // namespace hml

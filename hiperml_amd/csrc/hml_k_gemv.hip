@@ -1,0 +1,253 @@
+// hml_k_gemv.hip -- SGEMV on gfx950 (wave64, 128-bit loads, DPP reductions).
+//
+// Replaces the reference's three CUDA GEMV families BasicN / VarN / ConstNN
+// (include/hml_sgemv_kernel_template.h:19-106, 143-290, 328-411), which stage
+// A through shared memory and reduce with warp-synchronous volatile stores.
+// Here two forms cover every layout:
+//
+//  DOT  : y[m] = sum_n A[m*lda + n] x[n]  (each output a contiguous row).
+//         A wave splits into 64/LPR lane groups; a group streams one row
+//         with 128-bit loads, LPR lanes side by side, and reduces with DPP
+//         row operations (quad_perm, row_half_mirror, row_mirror) plus two
+//         cross-row shuffles.  Each lane group handles RB rows so one x load
+//         serves RB rows.  Optional split of n across waves with partials.
+//  AXPY : y[n] = sum_m A[m*lda + n] x[m]  (each output a column sum).
+//         LPR lanes x 4 floats span a column chunk; the 64/LPR lane groups
+//         and the waves of a block take interleaved rows; lane groups fold
+//         with shuffles, waves through LDS, and row splits through a
+//         fixed-order partial reduction (no atomics, deterministic).
+//
+// Neither form relies on out-of-range reads returning zero (the reference
+// relies on texture clamping, include/hml_sgemv_kernel_template.h:193-211):
+// every access is bounds-checked.
+#include "hml_internal.h"
+
+namespace hml {
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over aligned groups of LPR consecutive lanes; every lane of a group
+// ends with the group total.
+template <int LPR>
+__device__ __forceinline__ float group_sum(float v) {
+  if (LPR >= 2) v += dpp<0xB1>(v);    // quad_perm [1,0,3,2]
+  if (LPR >= 4) v += dpp<0x4E>(v);    // quad_perm [2,3,0,1]
+  if (LPR >= 8) v += dpp<0x141>(v);   // row_half_mirror
+  if (LPR >= 16) v += dpp<0x140>(v);  // row_mirror
+  if (LPR >= 32) v += __shfl_xor(v, 16);
+  if (LPR >= 64) v += __shfl_xor(v, 32);
+  return v;
+}
+
+template <bool VEC>
+__device__ __forceinline__ f4 load4(const float *p, int64_t nvalid) {
+  f4 v = {0.f, 0.f, 0.f, 0.f};
+  if (nvalid >= 4) {
+    if (VEC) {
+      v = *reinterpret_cast<const f4 *>(p);
+    } else {
+      v[0] = p[0]; v[1] = p[1]; v[2] = p[2]; v[3] = p[3];
+    }
+  } else if (nvalid > 0) {
+    v[0] = p[0];
+    if (nvalid > 1) v[1] = p[1];
+    if (nvalid > 2) v[2] = p[2];
+  }
+  return v;
+}
+
+template <int LPR, int RB, bool VEC>
+__global__ __launch_bounds__(256) void k_gemv_dot(GemvParams p) {
+  constexpr int G = 64 / LPR;
+  const int lane = threadIdx.x & 63;
+  const int wpb = blockDim.x >> 6;
+  const int64_t task = (int64_t)blockIdx.x * wpb + (threadIdx.x >> 6);
+  const int64_t rows_per_wave = (int64_t)G * RB;
+  const int64_t nrb = (p.M + rows_per_wave - 1) / rows_per_wave;
+  const int64_t rb = task / p.split;
+  const int sp = (int)(task - rb * p.split);
+  if (rb >= nrb) return;  // no barriers in this kernel
+  const int grp = lane / LPR, q = lane % LPR;
+
+  const float *arow[RB];
+  int64_t row[RB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i) {
+    row[i] = rb * rows_per_wave + (int64_t)i * G + grp;
+    const int64_t rc = row[i] < p.M ? row[i] : p.M - 1;
+    arow[i] = p.A + rc * p.lda;
+  }
+  const int64_t cb = (int64_t)sp * p.chunk;
+  int64_t ce = cb + p.chunk;
+  if (ce > p.N) ce = p.N;
+
+  float acc[RB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i) acc[i] = 0.f;
+
+  constexpr int STEP = 4 * LPR;
+#pragma unroll 2
+  for (int64_t c = cb + 4 * q; c < ce; c += STEP) {
+    const f4 xv = load4<VEC>(p.x + c, ce - c);
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      const f4 av = load4<VEC>(arow[i] + c, ce - c);
+      acc[i] += av[0] * xv[0];
+      acc[i] += av[1] * xv[1];
+      acc[i] += av[2] * xv[2];
+      acc[i] += av[3] * xv[3];
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < RB; ++i) {
+    const float s = group_sum<LPR>(acc[i]);
+    if (q == 0 && row[i] < p.M) {
+      if (p.split == 1) {
+        float *yp = p.y + row[i];
+        float v = p.alpha * s;
+        if (p.read_c) v = p.alpha * s + p.beta * (*yp);
+        *yp = v;
+      } else {
+        p.P[(int64_t)sp * p.M + row[i]] = s;
+      }
+    }
+  }
+}
+
+template <int LPR, bool VEC>
+__global__ __launch_bounds__(256) void k_gemv_axpy(GemvParams p) {
+  constexpr int G = 64 / LPR;
+  constexpr int CW = 4 * LPR;
+  __shared__ f4 red[4][LPR];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int wpb = blockDim.x >> 6;
+  const int grp = lane / LPR, q = lane % LPR;
+  const int64_t c0 = (int64_t)blockIdx.x * CW + 4 * q;
+  const int sp = blockIdx.y;
+  const int64_t rb = (int64_t)sp * p.chunk;
+  int64_t re = rb + p.chunk;
+  if (re > p.M) re = p.M;
+  const int64_t ncol = p.N - c0;
+
+  f4 acc = {0.f, 0.f, 0.f, 0.f};
+  const int64_t rstep = (int64_t)wpb * G;
+#pragma unroll 4
+  for (int64_t r = rb + (int64_t)w * G + grp; r < re; r += rstep) {
+    const float xr = p.x[r];
+    const f4 av = load4<VEC>(p.A + r * p.lda + c0, ncol);
+    acc += av * xr;
+  }
+#pragma unroll
+  for (int off = LPR; off < 64; off <<= 1) {
+    acc[0] += __shfl_xor(acc[0], off);
+    acc[1] += __shfl_xor(acc[1], off);
+    acc[2] += __shfl_xor(acc[2], off);
+    acc[3] += __shfl_xor(acc[3], off);
+  }
+  if (wpb > 1) {
+    if (grp == 0) red[w][q] = acc;
+    __syncthreads();
+    if (w == 0 && grp == 0) {
+      acc = red[0][q];
+      for (int i = 1; i < wpb; ++i) acc += red[i][q];
+    }
+  }
+  if (w == 0 && grp == 0) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int64_t col = c0 + t;
+      if (col < p.N) {
+        if (p.split == 1) {
+          float *yp = p.y + col;
+          float v = p.alpha * acc[t];
+          if (p.read_c) v = p.alpha * acc[t] + p.beta * (*yp);
+          *yp = v;
+        } else {
+          p.P[(int64_t)sp * p.N + col] = acc[t];
+        }
+      }
+    }
+  }
+}
+
+typedef hipError_t (*RunFn)(int, const GemvParams &, hipStream_t);
+
+template <int LPR, int RB, bool VEC>
+hipError_t run_dot(int wpb, const GemvParams &p, hipStream_t st) {
+  const int64_t rows = (int64_t)(64 / LPR) * RB;
+  const int64_t tasks = ((p.M + rows - 1) / rows) * p.split;
+  const int64_t nblk = (tasks + wpb - 1) / wpb;
+  if (nblk > 0x7fffffff) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL((k_gemv_dot<LPR, RB, VEC>), dim3((unsigned)nblk),
+                     dim3(64 * wpb), 0, st, p);
+  return hipGetLastError();
+}
+
+template <int LPR, bool VEC>
+hipError_t run_axpy(int wpb, const GemvParams &p, hipStream_t st) {
+  const int64_t ncb = (p.N + 4 * LPR - 1) / (4 * LPR);
+  if (ncb > 0x7fffffff || p.split > 65535) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL((k_gemv_axpy<LPR, VEC>), dim3((unsigned)ncb, (unsigned)p.split),
+                     dim3(64 * wpb), 0, st, p);
+  return hipGetLastError();
+}
+
+template <int LPR>
+RunFn dot_rb(int rb, bool vec) {
+  switch (rb) {
+    case 1: return vec ? run_dot<LPR, 1, true> : run_dot<LPR, 1, false>;
+    case 2: return vec ? run_dot<LPR, 2, true> : run_dot<LPR, 2, false>;
+    case 4: return vec ? run_dot<LPR, 4, true> : run_dot<LPR, 4, false>;
+    case 8: return vec ? run_dot<LPR, 8, true> : run_dot<LPR, 8, false>;
+    default: return nullptr;
+  }
+}
+
+RunFn find_gemv(int family, int lpr, int rb, bool vec) {
+  if (family == 1) {  // DOT
+    switch (lpr) {
+      case 4: return dot_rb<4>(rb, vec);
+      case 8: return dot_rb<8>(rb, vec);
+      case 16: return dot_rb<16>(rb, vec);
+      case 32: return dot_rb<32>(rb, vec);
+      case 64: return dot_rb<64>(rb, vec);
+      default: return nullptr;
+    }
+  } else if (family == 2) {  // AXPY; rb is unused (fixed unroll)
+    if (rb != 1) return nullptr;
+    switch (lpr) {
+      case 4: return vec ? run_axpy<4, true> : run_axpy<4, false>;
+      case 8: return vec ? run_axpy<8, true> : run_axpy<8, false>;
+      case 16: return vec ? run_axpy<16, true> : run_axpy<16, false>;
+      case 32: return vec ? run_axpy<32, true> : run_axpy<32, false>;
+      case 64: return vec ? run_axpy<64, true> : run_axpy<64, false>;
+      default: return nullptr;
+    }
+  }
+  return nullptr;
+}
+
+}  // namespace
+
+bool gemv_has_variant(int family, int lpr, int rb) {
+  return find_gemv(family, lpr, rb, true) != nullptr;
+}
+
+hipError_t launch_gemv(int family, int lpr, int rb, bool vec, int wpb,
+                       const GemvParams &p, hipStream_t st) {
+  RunFn fn = find_gemv(family, lpr, rb, vec);
+  if (!fn || (wpb != 1 && wpb != 2 && wpb != 4) || p.split < 1)
+    return hipErrorInvalidValue;
+  return fn(wpb, p, st);
+}
+
+}  // namespace hml

@@ -1,0 +1,233 @@
+// hml_runtime.cpp -- error state, device/memory helpers of the C ABI and the
+// library-owned workspace.
+//
+// The reference path reports every failure with fprintf + exit(1)
+// (src/hml_utils.cu:75-82 hmlHandleError, src/hml_sgemm.cu:189-199) and
+// allocates/frees device buffers inside every call (src/hml_utils.cu:147-230).
+// The extended ABI returns status codes instead and keeps one grow-only
+// scratch buffer per device.
+#include "hml_runtime.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <mutex>
+#include <vector>
+
+#include "hml_internal.h"
+
+namespace hml {
+
+namespace {
+thread_local std::string g_last_error;
+
+struct DeviceScratch {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_ws_mutex;
+std::vector<DeviceScratch> g_ws;
+
+std::mutex g_flush_mutex;
+std::vector<DeviceScratch> g_flush;
+}  // namespace
+
+hmlStatus fail(hmlStatus s, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return s;
+}
+
+hmlStatus from_hip(hipError_t e, const char *what) {
+  if (e == hipSuccess) return HML_STATUS_SUCCESS;
+  return fail(HML_STATUS_HIP_ERROR, "%s: %s", what, hipGetErrorString(e));
+}
+
+void clear_error() { g_last_error.clear(); }
+
+int verbosity() {
+  static int v = [] {
+    const char *s = getenv("HML_VERBOSE");
+    return s ? atoi(s) : 1;
+  }();
+  return v;
+}
+
+[[noreturn]] void compat_die(const char *where, hmlStatus s) {
+  fprintf(stderr, "%s: %s (%s)\n", where, hmlStatusString(s),
+          g_last_error.c_str());
+  exit(1);
+}
+
+hmlStatus library_workspace(size_t bytes, void **ptr) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return from_hip(e, "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_ws_mutex);
+  if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
+  DeviceScratch &w = g_ws[dev];
+  if (w.bytes < bytes) {
+    if (w.ptr) {
+      e = hipFree(w.ptr);
+      w.ptr = nullptr;
+      w.bytes = 0;
+      if (e != hipSuccess) return from_hip(e, "hipFree(workspace)");
+    }
+    size_t want = bytes < ((size_t)1 << 20) ? ((size_t)1 << 20) : bytes;
+    e = hipMalloc(&w.ptr, want);
+    if (e != hipSuccess) {
+      w.ptr = nullptr;
+      return fail(HML_STATUS_ALLOC_FAILED, "workspace hipMalloc(%zu): %s",
+                  want, hipGetErrorString(e));
+    }
+    w.bytes = want;
+  }
+  *ptr = w.ptr;
+  return HML_STATUS_SUCCESS;
+}
+
+}  // namespace hml
+
+using namespace hml;
+
+extern "C" {
+
+const char *hmlStatusString(hmlStatus s) {
+  switch (s) {
+    case HML_STATUS_SUCCESS: return "success";
+    case HML_STATUS_INVALID_VALUE: return "invalid value";
+    case HML_STATUS_HIP_ERROR: return "HIP runtime error";
+    case HML_STATUS_NO_KERNEL: return "no kernel for this variant";
+    case HML_STATUS_ALLOC_FAILED: return "allocation failed";
+    case HML_STATUS_IO_ERROR: return "I/O error";
+    case HML_STATUS_UNSUPPORTED: return "unsupported arguments";
+    case HML_STATUS_WORKSPACE_TOO_SMALL: return "workspace too small";
+    case HML_STATUS_VALIDATION_FAILED: return "validation failed";
+    case HML_STATUS_PARSE_ERROR: return "parse error";
+  }
+  return "unknown status";
+}
+
+const char *hmlLastError(void) { return g_last_error.c_str(); }
+
+hmlStatus hmlInit(void) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) return from_hip(e, "hipGetDeviceCount");
+  if (n <= 0) return fail(HML_STATUS_HIP_ERROR, "no GPU device");
+  return HML_STATUS_SUCCESS;
+}
+
+int hmlDeviceCount(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+hmlStatus hmlDeviceInfo(int device, char *name, int32_t *cus,
+                        int64_t *l2_bytes, int64_t *hbm_bytes,
+                        int32_t *clock_khz) {
+  hipDeviceProp_t prop;
+  hipError_t e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) return from_hip(e, "hipGetDeviceProperties");
+  if (name) {
+    snprintf(name, 64, "%s", prop.gcnArchName);
+  }
+  if (cus) *cus = prop.multiProcessorCount;
+  if (l2_bytes) *l2_bytes = prop.l2CacheSize;
+  if (hbm_bytes) *hbm_bytes = (int64_t)prop.totalGlobalMem;
+  if (clock_khz) *clock_khz = prop.clockRate;
+  return HML_STATUS_SUCCESS;
+}
+
+hmlStatus hmlDeviceMalloc(void **ptr, size_t bytes) {
+  if (!ptr) return fail(HML_STATUS_INVALID_VALUE, "ptr is NULL");
+  hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+  if (e != hipSuccess)
+    return fail(HML_STATUS_ALLOC_FAILED, "hipMalloc(%zu): %s", bytes,
+                hipGetErrorString(e));
+  return HML_STATUS_SUCCESS;
+}
+
+hmlStatus hmlDeviceFree(void *ptr) { return from_hip(hipFree(ptr), "hipFree"); }
+
+hmlStatus hmlMemcpyHtoD(void *dst, const void *src, size_t bytes) {
+  return from_hip(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+hmlStatus hmlMemcpyDtoH(void *dst, const void *src, size_t bytes) {
+  return from_hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+hmlStatus hmlMemcpyDtoD(void *dst, const void *src, size_t bytes) {
+  return from_hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
+}
+hmlStatus hmlMemsetD8(void *dst, int value, size_t bytes) {
+  return from_hip(hipMemset(dst, value, bytes), "hipMemset");
+}
+hmlStatus hmlDeviceSynchronize(void) {
+  return from_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}
+hmlStatus hmlStreamCreate(void **stream) {
+  hipStream_t s;
+  hipError_t e = hipStreamCreate(&s);
+  if (e != hipSuccess) return from_hip(e, "hipStreamCreate");
+  *stream = s;
+  return HML_STATUS_SUCCESS;
+}
+hmlStatus hmlStreamDestroy(void *stream) {
+  return from_hip(hipStreamDestroy((hipStream_t)stream), "hipStreamDestroy");
+}
+hmlStatus hmlStreamSynchronize(void *stream) {
+  return from_hip(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+}
+hmlStatus hmlEventCreate(void **event) {
+  hipEvent_t ev;
+  hipError_t e = hipEventCreate(&ev);
+  if (e != hipSuccess) return from_hip(e, "hipEventCreate");
+  *event = ev;
+  return HML_STATUS_SUCCESS;
+}
+hmlStatus hmlEventDestroy(void *event) {
+  return from_hip(hipEventDestroy((hipEvent_t)event), "hipEventDestroy");
+}
+hmlStatus hmlEventRecord(void *event, void *stream) {
+  return from_hip(hipEventRecord((hipEvent_t)event, (hipStream_t)stream), "hipEventRecord");
+}
+hmlStatus hmlEventSynchronize(void *event) {
+  return from_hip(hipEventSynchronize((hipEvent_t)event), "hipEventSynchronize");
+}
+hmlStatus hmlEventElapsedMs(float *ms, void *start, void *stop) {
+  return from_hip(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop),
+                  "hipEventElapsedTime");
+}
+
+hmlStatus hmlFlushCaches(void *stream) {
+  // 512 MiB of writes evicts the 256 MB MALL and every XCD's L2.
+  const size_t bytes = (size_t)512 << 20;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return from_hip(e, "hipGetDevice");
+  void *p = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_flush_mutex);
+    if ((int)g_flush.size() <= dev) g_flush.resize(dev + 1);
+    if (!g_flush[dev].ptr) {
+      e = hipMalloc(&g_flush[dev].ptr, bytes);
+      if (e != hipSuccess) {
+        g_flush[dev].ptr = nullptr;
+        return fail(HML_STATUS_ALLOC_FAILED, "flush buffer: %s", hipGetErrorString(e));
+      }
+      g_flush[dev].bytes = bytes;
+    }
+    p = g_flush[dev].ptr;
+  }
+  return from_hip(launch_fill((float *)p, bytes / sizeof(float), 0.f,
+                              (hipStream_t)stream),
+                  "flush fill");
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+// hml_select.h -- canonical problem descriptions and the variant selectors
+// (built-in heuristic, config store, learned MLP) shared by the GEMM/GEMV
+// dispatch, the autotuner and the config store.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/hml_ext.h"
+
+namespace hml {
+
+// A GEMM after layout canonicalisation and orientation:
+// C(m,n) = sum_k A(m,k) B(k,n); A is the streamed (larger) operand.
+struct GemmCanon {
+  const float *A; int64_t sAm, sAk;
+  const float *B; int64_t sBk, sBn;
+  float *C; int64_t sCm, sCn;
+  int64_t M, N, K;
+  int form;      // kFormDot / kFormAxpy
+  bool swapped;  // m/n roles exchanged relative to the caller's C
+};
+
+// Shape key used by every selector.
+struct GemmKey {
+  int form;      // 0 dot, 1 axpy (after orientation)
+  int64_t M, N, K;
+  int beta0;     // beta == 0 and C is not read
+  int transA, transB, order;  // caller's layout (features of the learned selector)
+  bool gemv_ok;  // N == 1 and x / y are unit-stride, GEMV kernels apply
+};
+
+struct GemvKey {
+  int family;    // 1 DOT, 2 AXPY
+  int64_t outputs, reduced;
+  int beta0;
+};
+
+// Layout -> strides, orientation (larger operand streamed), form.
+GemmCanon canonicalize(hmlOrder order, hmlOp ta, hmlOp tb, int64_t M,
+                       int64_t N, int64_t K, const float *A, int64_t lda,
+                       const float *B, int64_t ldb, float *C, int64_t ldc);
+GemmKey make_key(const GemmCanon &g, float beta, uint32_t flags,
+                 hmlOrder order, hmlOp ta, hmlOp tb);
+GemmCanon dense_canon(hmlOrder order, hmlOp ta, hmlOp tb, int64_t M,
+                      int64_t N, int64_t K);
+// SGEMV layout -> family and extents.
+GemvKey gemv_key(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
+                 int beta0);
+int ilog2_floor(int64_t x);
+
+void heuristic_sgemm(const GemmKey &k, hmlSgemmVariant *v);
+void heuristic_sgemv(const GemvKey &k, hmlSgemvVariant *v);
+void basic_sgemm(const GemmKey &k, hmlSgemmVariant *v);
+void basic_sgemv(const GemvKey &k, hmlSgemvVariant *v);
+
+// Normalises a variant so it can run (ct divides the padded tile count,
+// ksplit at least 1, ...).  Returns false when the variant cannot run this
+// key at all (wrong family for the form, no instantiation).
+bool sgemm_variant_runnable(const GemmKey &k, const hmlSgemmVariant &v);
+bool sgemv_variant_runnable(const GemvKey &k, const hmlSgemvVariant &v);
+
+// Config-store / learned selection; return false when they have no answer.
+bool config_select_sgemm(const GemmKey &k, hmlSgemmVariant *v, int *source);
+bool config_select_sgemv(const GemvKey &k, hmlSgemvVariant *v, int *source);
+
+// Selection entry used by the dispatch (config -> heuristic).
+void select_sgemm(const GemmKey &k, hmlSgemmVariant *v, int *source);
+void select_sgemv(const GemvKey &k, hmlSgemvVariant *v, int *source);
+
+}  // namespace hml

@@ -1,0 +1,249 @@
+/*
+ * hml_ext.h -- extended C-ABI of the MI355X skinny SGEMM/SGEMV library.
+ *
+ * The reference exposes only the host-pointer entry points of hml_sgemm.h /
+ * hml_sgemv.h, plus C++ internals that its apps call directly
+ * (include/hml_sgemm_kernel.h:68-101, include/hml_sgemv_kernel.h:72-139).
+ * Those internals hand out CUDA __global__ function pointers and cannot be a
+ * C ABI; this header replaces them with plain-pointer, status-returning
+ * calls.  Every pointer argument named A/B/C/x/y/ws is a DEVICE pointer
+ * unless stated otherwise; `stream` is a hipStream_t passed as void* (NULL =
+ * the null stream).  Calls are asynchronous with respect to the host unless
+ * stated otherwise.
+ */
+#ifndef HML_EXT_H_INCLUDED_
+#define HML_EXT_H_INCLUDED_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference path has none: it exits) ------------- */
+typedef enum {
+  HML_STATUS_SUCCESS             = 0,
+  HML_STATUS_INVALID_VALUE       = 1,
+  HML_STATUS_HIP_ERROR           = 2,
+  HML_STATUS_NO_KERNEL           = 3,
+  HML_STATUS_ALLOC_FAILED        = 4,
+  HML_STATUS_IO_ERROR            = 5,
+  HML_STATUS_UNSUPPORTED         = 6,
+  HML_STATUS_WORKSPACE_TOO_SMALL = 7,
+  HML_STATUS_VALIDATION_FAILED   = 8,
+  HML_STATUS_PARSE_ERROR         = 9
+} hmlStatus;
+
+typedef enum { HML_ROW_MAJOR = 0, HML_COL_MAJOR = 1 } hmlOrder;
+typedef enum { HML_OP_N = 0, HML_OP_T = 1 } hmlOp;
+
+/* flags for the Ex entry points */
+#define HML_FLAG_READ_C_ALWAYS 0x1u /* read C (or y) even when beta == 0, as
+                                       the reference kernels do; without it
+                                       beta == 0 never reads C (BLAS rule). */
+
+/* ---- kernel variants ----------------------------------------------------
+ * Replaces HmlSgemmKernelType/Config (include/hml_sgemm_kernel.h:33-43) and
+ * HmlSgemvKernelType/Config (include/hml_sgemv_kernel.h:38-49): a variant is
+ * a (family, tile parameters, split) tuple instead of a function pointer. */
+typedef enum {
+  HML_SGEMM_FAMILY_BASIC  = 0, /* fixed default variant                   */
+  HML_SGEMM_FAMILY_DOT    = 1, /* MFMA f32, streamed operand contiguous in K */
+  HML_SGEMM_FAMILY_AXPY   = 2, /* MFMA f32, streamed operand contiguous in M */
+  HML_SGEMM_FAMILY_GEMV   = 3, /* N == 1 routed to the GEMV kernels        */
+  HML_SGEMM_FAMILY_BF16X3 = 4, /* split-precision bf16 MFMA (dense)        */
+  HML_SGEMM_FAMILY_DENSE  = 5  /* plain FP32 MFMA dense tile kernel        */
+} hmlSgemmFamily;
+
+typedef struct {
+  int32_t family; /* hmlSgemmFamily                                          */
+  int32_t rt;     /* row tiles per wave (16 rows DOT, 64 rows AXPY)           */
+  int32_t ct;     /* column tiles of 16 per wave                              */
+  int32_t ksplit; /* K split across workgroups (partials + reduce), >= 1      */
+  int32_t wpb;    /* waves per workgroup (1, 2 or 4)                          */
+} hmlSgemmVariant;
+
+typedef enum {
+  HML_SGEMV_FAMILY_BASIC = 0,
+  HML_SGEMV_FAMILY_DOT   = 1, /* outputs are dot products of contiguous rows */
+  HML_SGEMV_FAMILY_AXPY  = 2  /* outputs are sums of contiguous rows         */
+} hmlSgemvFamily;
+
+typedef struct {
+  int32_t family; /* hmlSgemvFamily                                      */
+  int32_t lpr;    /* lanes per row (DOT) / lanes per column chunk (AXPY)  */
+  int32_t rb;     /* rows per lane group (DOT) / unroll (AXPY)            */
+  int32_t split;  /* split of the reduced dimension across workgroups     */
+  int32_t wpb;    /* waves per workgroup                                  */
+} hmlSgemvVariant;
+
+/* ---- GEMM / GEMV on device pointers --------------------------------------
+ * C = alpha * op(A) * op(B) + beta * C, C is M x N, op(A) M x K, op(B) K x N,
+ * in `order` with leading dimensions lda/ldb/ldc (BLAS convention).
+ * workspace may be NULL (the library then uses a per-device buffer it owns
+ * and grows on demand; such calls must not be captured into a graph the
+ * first time a larger buffer is needed); otherwise ws_bytes must be at least
+ * hmlSgemmWorkspaceSize() for the same arguments. */
+hmlStatus hmlSgemmEx(hmlOrder order, hmlOp transA, hmlOp transB,
+                     int64_t M, int64_t N, int64_t K, float alpha,
+                     const float *A, int64_t lda, const float *B, int64_t ldb,
+                     float beta, float *C, int64_t ldc, uint32_t flags,
+                     void *workspace, size_t ws_bytes, void *stream);
+
+/* Same with an explicit variant (autotuner, tests).  variant->family must
+ * match the layout (DOT needs op(A) contiguous along K after orientation,
+ * AXPY contiguous along M); a mismatched or invalid variant returns
+ * HML_STATUS_NO_KERNEL. */
+hmlStatus hmlSgemmExVariant(hmlOrder order, hmlOp transA, hmlOp transB,
+                            int64_t M, int64_t N, int64_t K, float alpha,
+                            const float *A, int64_t lda, const float *B,
+                            int64_t ldb, float beta, float *C, int64_t ldc,
+                            uint32_t flags, const hmlSgemmVariant *variant,
+                            void *workspace, size_t ws_bytes, void *stream);
+
+/* Workspace bytes needed by hmlSgemmEx (variant == NULL: the variant the
+ * selector would choose) or by hmlSgemmExVariant. */
+size_t hmlSgemmWorkspaceSize(hmlOrder order, hmlOp transA, hmlOp transB,
+                             int64_t M, int64_t N, int64_t K,
+                             const hmlSgemmVariant *variant);
+
+/* The variant hmlSgemmEx would run for these arguments, and which selector
+ * produced it (0 built-in heuristic, 1 config table, 2 learned MLP). */
+hmlStatus hmlSgemmSelect(hmlOrder order, hmlOp transA, hmlOp transB,
+                         int64_t M, int64_t N, int64_t K, float beta,
+                         hmlSgemmVariant *variant, int32_t *source);
+
+/* y = alpha * op(A) * x + beta * y; A is M x N in `order`; incx/incy must
+ * be 1. */
+hmlStatus hmlSgemvEx(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
+                     float alpha, const float *A, int64_t lda, const float *x,
+                     int64_t incx, float beta, float *y, int64_t incy,
+                     uint32_t flags, void *workspace, size_t ws_bytes,
+                     void *stream);
+
+hmlStatus hmlSgemvExVariant(hmlOrder order, hmlOp trans, int64_t M,
+                            int64_t N, float alpha, const float *A,
+                            int64_t lda, const float *x, int64_t incx,
+                            float beta, float *y, int64_t incy,
+                            uint32_t flags, const hmlSgemvVariant *variant,
+                            void *workspace, size_t ws_bytes, void *stream);
+
+size_t hmlSgemvWorkspaceSize(hmlOrder order, hmlOp trans, int64_t M,
+                             int64_t N, const hmlSgemvVariant *variant);
+
+hmlStatus hmlSgemvSelect(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
+                         float beta, hmlSgemvVariant *variant,
+                         int32_t *source);
+
+/* ---- config store ----------------------------------------------------------
+ * Replaces hmlSgemmKernelConfigReadFile / ConfigPrint
+ * (src/hml_sgemm_kernel.cu:557-602) and the SGEMV pair
+ * (src/hml_sgemv_kernel.cu:634-679) with status-returning calls.  Loading a
+ * v1.0 file keeps the reference's table (K 0..96 x colStops 0..16, N
+ * 0..1024); a v2.0 file (MI355X) holds variants keyed by power-of-two shape
+ * buckets.  Save writes v2.0 when v2 rules are active, else v1.0. */
+hmlStatus hmlSgemmConfigLoad(const char *path);
+hmlStatus hmlSgemmConfigSave(const char *path);
+hmlStatus hmlSgemvConfigLoad(const char *path);
+hmlStatus hmlSgemvConfigSave(const char *path);
+void      hmlConfigClear(void);
+/* v1 table entry; returns 1 if a v1 table is loaded, else 0. */
+int       hmlSgemmConfigV1Get(int K, int colStops, int *type, int *rowStops);
+int       hmlSgemvConfigV1Get(int N, int *type, int *blockStops, int *rowStops);
+int64_t   hmlSgemmConfigRuleCount(void); /* active v2 rules */
+int64_t   hmlSgemvConfigRuleCount(void);
+
+/* ---- exhaustive autotuner -------------------------------------------------
+ * Replaces hmlSgemmKernelConfigOnline (src/hml_sgemm_kernel.cu:356-555) and
+ * hmlSgemvKernelConfigOnline (src/hml_sgemv_kernel.cu:391-632) for one
+ * shape: every candidate variant is validated against a reference output
+ * (itself checked by the FP64 kernel), then timed `trials` times with
+ * hipEvents (after a cache flush when `flush`), and the median decides.
+ * Each (shape, variant, median ms) is appended to the CSV `db_path` (may be
+ * NULL).  heuristic_ms receives the built-in heuristic variant's time. */
+typedef struct {
+  int64_t M, N, K;
+  int32_t order, transA, transB, beta0;
+} hmlGemmShape;
+
+/* Optional replacement of GPU timing (for host-only tests of the search):
+ * returns milliseconds, or < 0 for a failed candidate.  kind 0 = SGEMM
+ * (dims = M,N,K; variant = family,rt,ct,ksplit,wpb), 1 = SGEMV (dims =
+ * M,N,0; variant = family,lpr,rb,split,wpb).  NULL restores GPU timing. */
+typedef double (*hmlAutotuneTimer)(int32_t kind, const int64_t *dims,
+                                   const int32_t *variant, void *user);
+void    hmlAutotuneSetTimer(hmlAutotuneTimer fn, void *user);
+
+int32_t hmlSgemmCandidates(const hmlGemmShape *shape, hmlSgemmVariant *out,
+                           int32_t cap);
+int32_t hmlSgemvCandidates(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
+                           hmlSgemvVariant *out, int32_t cap);
+hmlStatus hmlSgemmAutotune(const hmlGemmShape *shape, int32_t trials,
+                           int32_t flush, uint64_t seed, hmlSgemmVariant *best,
+                           double *best_ms, double *heuristic_ms,
+                           const char *db_path);
+hmlStatus hmlSgemvAutotune(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
+                           int32_t trials, int32_t flush, uint64_t seed,
+                           hmlSgemvVariant *best, double *best_ms,
+                           double *heuristic_ms, const char *db_path);
+/* Install (or replace) the v2 rule of the shape's bucket. */
+hmlStatus hmlSgemmConfigAddRule(const hmlGemmShape *shape,
+                                const hmlSgemmVariant *variant);
+hmlStatus hmlSgemvConfigAddRule(hmlOrder order, hmlOp trans, int64_t M,
+                                int64_t N, const hmlSgemvVariant *variant);
+
+/* dst[i] = lo + (hi - lo) * u_i with u_i from a counter-based splitmix64
+ * stream of `seed` (device-side synthetic inputs). */
+hmlStatus hmlFillUniform(float *dst, size_t n, uint64_t seed, float lo,
+                         float hi, void *stream);
+
+/* ---- FP64 check kernel ----------------------------------------------------
+ * Computes R = alpha * op(A) op(B) + beta * C0 in FP64 on the GPU and
+ * compares it with the FP32 result C.  stats (host, 6 doubles):
+ *   [0] sum (C - R)^2          [1] sum R^2
+ *   [2] sum (|alpha| |op(A)||op(B)| + |beta| |C0|)^2   (for the gamma_K bound)
+ *   [3] max |C - R|            [4] max |C - R| / (|alpha||A||B|+|beta||C0|)
+ *   [5] count of non-finite C where R is finite
+ * C0 may be NULL when beta == 0.  Synchronous. */
+hmlStatus hmlSgemmCheckF64(hmlOrder order, hmlOp transA, hmlOp transB,
+                           int64_t M, int64_t N, int64_t K, float alpha,
+                           const float *A, int64_t lda, const float *B,
+                           int64_t ldb, float beta, const float *C0,
+                           const float *C, int64_t ldc, double *stats);
+
+/* ---- device / memory helpers (so hosts need no other GPU runtime) -------- */
+hmlStatus hmlInit(void);
+int       hmlDeviceCount(void);
+/* name (>= 64 bytes), compute units, L2 bytes, HBM bytes, clock kHz */
+hmlStatus hmlDeviceInfo(int device, char *name, int32_t *cus,
+                        int64_t *l2_bytes, int64_t *hbm_bytes,
+                        int32_t *clock_khz);
+hmlStatus hmlDeviceMalloc(void **ptr, size_t bytes);
+hmlStatus hmlDeviceFree(void *ptr);
+hmlStatus hmlMemcpyHtoD(void *dst, const void *src, size_t bytes);
+hmlStatus hmlMemcpyDtoH(void *dst, const void *src, size_t bytes);
+hmlStatus hmlMemcpyDtoD(void *dst, const void *src, size_t bytes);
+hmlStatus hmlMemsetD8(void *dst, int value, size_t bytes);
+hmlStatus hmlDeviceSynchronize(void);
+hmlStatus hmlStreamCreate(void **stream);
+hmlStatus hmlStreamDestroy(void *stream);
+hmlStatus hmlStreamSynchronize(void *stream);
+hmlStatus hmlEventCreate(void **event);
+hmlStatus hmlEventDestroy(void *event);
+hmlStatus hmlEventRecord(void *event, void *stream);
+hmlStatus hmlEventSynchronize(void *event);
+hmlStatus hmlEventElapsedMs(float *ms, void *start, void *stop);
+/* Writes `bytes` of junk through the caches so the next timed kernel sees
+ * cold L2/MALL (used by the autotuner and benchmarks). */
+hmlStatus hmlFlushCaches(void *stream);
+
+const char *hmlStatusString(hmlStatus s);
+/* Thread-local text of the last failing call (empty if none). */
+const char *hmlLastError(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HML_EXT_H_INCLUDED_ */

@@ -1,0 +1,23 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (ROOT, os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box)")
+
+
+@pytest.fixture(scope="session")
+def hml():
+    """The product library; GPU tests fail (not skip) if it cannot load or
+    no device is visible."""
+    import hiperml_amd
+    assert hiperml_amd.device_count() > 0, "no GPU visible to libhml.so"
+    hiperml_amd.check(hiperml_amd.lib.hmlInit())
+    return hiperml_amd
