@@ -50,6 +50,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time budget of the baseline sample")
+    ap.add_argument("--dense", action="store_true",
+                    help="also run the split-precision dense suite (extra JSON line)")
+    ap.add_argument("--grouped", action="store_true",
+                    help="also run the grouped 512-problem suite (extra JSON line)")
+    ap.add_argument("--only-extra", action="store_true",
+                    help="skip the main sweep, run only the requested extra suites")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU "
@@ -61,6 +67,12 @@ def main():
     import hiperml_amd as hml
 
     hml.check(hml.lib.hmlInit())
+    if args.only_extra:
+        if args.dense:
+            bench_dense(hml)
+        if args.grouped:
+            bench_grouped(hml)
+        return
     stream = hml.Stream()
     S = stream.ptr
 
@@ -197,6 +209,119 @@ def main():
         "shapes": shapes,
     }
     print(json.dumps(out))
+    sys.stdout.flush()
+    if args.dense:
+        bench_dense(hml)
+    if args.grouped:
+        bench_grouped(hml)
+
+
+def bench_dense(hml, reps=5):
+    """Split-precision bf16x3 / bf16x6 vs plain FP32 on the matrix cores:
+    M = N = K = 8192 and M = N = 8192, K = 256, inputs uniform[-1,1] (set A)
+    and log-uniform magnitudes in [1e-3, 1e3] with random sign (set B);
+    error against the FP64 check kernel."""
+    import numpy as np
+    stream = hml.Stream()  # keep the object: its destructor destroys the stream
+    S = stream.ptr
+    rows = []
+    for (M, N, K) in ((8192, 8192, 8192), (8192, 8192, 256)):
+        rng = np.random.default_rng(M + K)
+        for dist in ("uniform", "loguniform"):
+            if dist == "uniform":
+                A = rng.uniform(-1, 1, (M, K)).astype(np.float32)
+                B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+            else:
+                A = (10.0 ** rng.uniform(-3, 3, (M, K)) * rng.choice([-1.0, 1.0], (M, K))).astype(np.float32)
+                B = (10.0 ** rng.uniform(-3, 3, (K, N)) * rng.choice([-1.0, 1.0], (K, N))).astype(np.float32)
+            dA, dB = hml.DeviceBuffer.from_numpy(A), hml.DeviceBuffer.from_numpy(B)
+            dC = hml.DeviceBuffer(4 * M * N)
+            del A, B
+            for prec in ("fp32", "bf16x3", "bf16x6"):
+                ws = hml.lib.hmlSgemmDenseWorkspaceSize(hml.PRECISION[prec], M, N, K)
+                W = hml.DeviceBuffer(ws)
+                run = lambda: hml.sgemm_dense(M, N, K, dA, dB, dC, precision=prec,
+                                              workspace=W, ws_bytes=ws, stream=S)
+                run()
+                hml.synchronize()
+                e0, e1 = hml.Event(), hml.Event()
+                e0.record(S)
+                for _ in range(reps):
+                    run()
+                e1.record(S)
+                e1.synchronize()
+                ms = e0.elapsed_ms(e1) / reps
+                st = hml.check_f64(M, N, K, dA, dB, dC)
+                rows.append({"M": M, "N": N, "K": K, "inputs": dist, "precision": prec,
+                             "ms": round(ms, 4), "TFLOP_s": round(2.0 * M * N * K / ms / 1e9, 2),
+                             "rel_fro_err_fp64": st["rel_fro"],
+                             "max_err_over_bound": st["max_rel_bound"]})
+                del W
+    for r in rows:
+        base = [q for q in rows if q["M"] == r["M"] and q["K"] == r["K"] and
+                q["inputs"] == r["inputs"] and q["precision"] == "fp32"][0]
+        r["speedup_vs_fp32"] = round(base["ms"] / r["ms"], 3)
+        r["err_vs_fp32"] = round(r["rel_fro_err_fp64"] / base["rel_fro_err_fp64"], 3)
+    print(json.dumps({"suite": "dense split-precision (bf16x3/bf16x6) vs plain FP32 MFMA",
+                      "peaks": {"fp32_TFLOP_s": FP32_PEAK_TFLOPS, "bf16_dense_TFLOP_s": 2516.6},
+                      "rows": rows}))
+
+
+def bench_grouped(hml, count=512, reps=5, seed=0):
+    """512 uneven skinny problems (M log-uniform [64,65536], K log-uniform
+    [64,4096], N uniform in 1..64, uniform[-1,1]) in one persistent grouped
+    launch (direct and hipGraph) vs 512 sequential hmlSgemmEx calls."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    S = hml.Stream()
+    probs, nbytes, flops = [], 0.0, 0.0
+    for i in range(count):
+        M = int(round(np.exp(rng.uniform(np.log(64), np.log(65536)))))
+        K = int(round(np.exp(rng.uniform(np.log(64), np.log(4096)))))
+        N = int(rng.integers(1, 65))
+        A = hml.DeviceBuffer(4 * M * K); A.fill_uniform(M * K, 3 * i + 1, stream=S.ptr)
+        B = hml.DeviceBuffer(4 * K * N); B.fill_uniform(K * N, 3 * i + 2, stream=S.ptr)
+        C = hml.DeviceBuffer(4 * M * N)
+        probs.append((M, N, K, A, B, C))
+        nbytes += 4.0 * (M * K + K * N + M * N)
+        flops += 2.0 * M * N * K
+    ws = max(hml.lib.hmlSgemmWorkspaceSize(0, 0, 0, p[0], p[1], p[2], None) for p in probs)
+    W = hml.DeviceBuffer(ws)
+    plan = hml.GroupedPlan(probs)
+
+    def seq():
+        for M, N, K, A, B, C in probs:
+            hml.sgemm_ex(M, N, K, A, B, C, workspace=W, ws_bytes=ws, stream=S.ptr)
+
+    def timed(fn):
+        fn()
+        S.synchronize()
+        e0, e1 = hml.Event(), hml.Event()
+        e0.record(S.ptr)
+        for _ in range(reps):
+            fn()
+        e1.record(S.ptr)
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / reps
+
+    t_seq = timed(seq)
+    t_dir = timed(lambda: plan.run(graph=False, stream=S.ptr))
+    t_graph = timed(lambda: plan.run(graph=True, stream=S.ptr))
+    info = plan.info()
+    # accuracy of the grouped result: worst problem against FP64
+    worst = 0.0
+    for M, N, K, A, B, C in probs[:64]:
+        st = hml.check_f64(M, N, K, A, B, C)
+        worst = max(worst, st["max_rel_bound"])
+    plan.destroy()
+    print(json.dumps({"suite": "grouped skinny SGEMM, %d problems" % count,
+                      "algorithmic_GB": round(nbytes / 1e9, 3), "GFLOP": round(flops / 1e9, 1),
+                      "sequential_ms": round(t_seq, 4), "grouped_ms": round(t_dir, 4),
+                      "grouped_graph_ms": round(t_graph, 4),
+                      "speedup_graph_vs_sequential": round(t_seq / t_graph, 3),
+                      "grouped_graph_GB_s": round(nbytes / t_graph / 1e6, 1),
+                      "tiles": info["tiles"], "blocks": info["blocks"],
+                      "max_err_over_bound_first64": worst}))
 
 
 def cpu_baseline(hml, calls, budget_s):

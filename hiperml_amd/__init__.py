@@ -204,6 +204,184 @@ def sgemv_ex(M, N, A, x, y, alpha=1.0, beta=0.0, order="row", trans=False,
     check(st)
 
 
+PRECISION = {"fp32": 0, "bf16x3": 1, "bf16x6": 2}
+
+
+def sgemm_dense(M, N, K, A, B, C, alpha=1.0, beta=0.0, precision="fp32",
+                order="row", transA=False, transB=False, lda=None, ldb=None,
+                ldc=None, flags=0, workspace=None, ws_bytes=0, stream=None):
+    """Dense SGEMM on the matrix cores (hmlSgemmDense): precision 'fp32'
+    (plain FP32 MFMA), 'bf16x3' or 'bf16x6' (split-precision bf16 MFMA)."""
+    o, ta, tb = _order(order), _op(transA), _op(transB)
+    if lda is None:
+        lda = _ld(o, M, K) if ta == OP_N else _ld(o, K, M)
+    if ldb is None:
+        ldb = _ld(o, K, N) if tb == OP_N else _ld(o, N, K)
+    if ldc is None:
+        ldc = _ld(o, M, N)
+    check(lib.hmlSgemmDense(PRECISION[precision], o, ta, tb, M, N, K, alpha,
+                            _ptr(A), lda, _ptr(B), ldb, beta, _ptr(C), ldc,
+                            flags, _ptr(workspace), ws_bytes, stream))
+
+
+class GroupedPlan:
+    """hmlSgemmGroupedCreate/Run/Destroy: many row-major problems
+    C_i = alpha A_i B_i + beta C_i in one persistent launch.  `problems` is a
+    list of (M, N, K, A, B, C) with device buffers (dense leading dims) or
+    (M, N, K, A, lda, B, ldb, C, ldc)."""
+
+    def __init__(self, problems, alpha=1.0, beta=0.0):
+        n = len(problems)
+        I64 = ctypes.c_int64 * n
+        VP = ctypes.c_void_p * n
+        Ms, Ns, Ks, As, las, Bs, lbs, Cs, lcs = ([] for _ in range(9))
+        for pr in problems:
+            if len(pr) == 6:
+                M, N, K, A, B, C = pr
+                la, lb, lc = K, N, N
+            else:
+                M, N, K, A, la, B, lb, C, lc = pr
+            Ms.append(M); Ns.append(N); Ks.append(K)
+            As.append(_ptr(A)); Bs.append(_ptr(B)); Cs.append(_ptr(C))
+            las.append(la); lbs.append(lb); lcs.append(lc)
+        h = ctypes.c_void_p()
+        check(lib.hmlSgemmGroupedCreate(ctypes.byref(h), n, I64(*Ms), I64(*Ns),
+                                        I64(*Ks), VP(*As), I64(*las), VP(*Bs),
+                                        I64(*lbs), VP(*Cs), I64(*lcs), alpha,
+                                        beta))
+        self.ptr = h.value
+
+    def run(self, graph=False, stream=None):
+        check(lib.hmlSgemmGroupedRun(self.ptr, 1 if graph else 0, stream))
+
+    def info(self):
+        nt = ctypes.c_int64()
+        nb = ctypes.c_int32()
+        by = ctypes.c_size_t()
+        check(lib.hmlSgemmGroupedInfo(self.ptr, ctypes.byref(nt), ctypes.byref(nb),
+                                      ctypes.byref(by)))
+        return {"tiles": nt.value, "blocks": nb.value, "device_bytes": by.value}
+
+    def destroy(self):
+        if self.ptr:
+            lib.hmlSgemmGroupedDestroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Mlp:
+    """hmlMlp*: the learned selector's network, trained and evaluated by
+    HIP kernels.  Arrays are numpy float32 (host)."""
+
+    def __init__(self, n_in=6, hidden=64, n_hidden=3, n_out=None, seed=0,
+                 _ptr=None):
+        if _ptr is not None:
+            self.ptr = _ptr
+            return
+        if n_out is None:
+            n_out = lib.hmlSelectorClassCount()
+        h = ctypes.c_void_p()
+        check(lib.hmlMlpCreate(ctypes.byref(h), n_in, hidden, n_hidden, n_out, seed))
+        self.ptr = h.value
+
+    @classmethod
+    def load(cls, path):
+        h = ctypes.c_void_p()
+        check(lib.hmlMlpLoad(ctypes.byref(h), path.encode()))
+        return cls(_ptr=h.value)
+
+    def save(self, path):
+        check(lib.hmlMlpSave(self.ptr, path.encode()))
+
+    def nparams(self):
+        return lib.hmlMlpParamCount(self.ptr)
+
+    def get_params(self):
+        out = np.empty(self.nparams(), np.float32)
+        check(lib.hmlMlpGetParams(self.ptr, _fp(out), out.size))
+        return out
+
+    def set_params(self, p):
+        p = _f32(p)
+        check(lib.hmlMlpSetParams(self.ptr, _fp(p), p.size))
+
+    def loss_grad(self, X, T, mask):
+        X, T, mask = _f32(X), _f32(T), _f32(mask)
+        g = np.empty(self.nparams(), np.float32)
+        loss = ctypes.c_float()
+        check(lib.hmlMlpLossGrad(self.ptr, _fp(X), _fp(T), _fp(mask), X.shape[0],
+                                 ctypes.byref(loss), _fp(g)))
+        return loss.value, g
+
+    def train(self, X, T, mask, epochs=10, batch=256, lr=1e-3, seed=0):
+        X, T, mask = _f32(X), _f32(T), _f32(mask)
+        losses = np.empty(epochs, np.float32)
+        check(lib.hmlMlpTrain(self.ptr, _fp(X), _fp(T), _fp(mask), X.shape[0],
+                              epochs, batch, lr, seed, _fp(losses)))
+        return losses
+
+    def predict(self, X, mask):
+        X, mask = _f32(X), _f32(mask)
+        P = np.empty(mask.shape, np.float32)
+        check(lib.hmlMlpPredict(self.ptr, _fp(X), _fp(mask), X.shape[0], _fp(P)))
+        return P
+
+    def destroy(self):
+        if self.ptr:
+            lib.hmlMlpDestroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def _shape(M, N, K, order="row", transA=False, transB=False, beta0=True):
+    return GemmShape(M, N, K, _order(order), _op(transA), _op(transB), 1 if beta0 else 0)
+
+
+def selector_features(M, N, K, **kw):
+    f = np.empty(lib.hmlSelectorFeatureCount(), np.float32)
+    check(lib.hmlSelectorFeatures(ctypes.byref(_shape(M, N, K, **kw)), _fp(f)))
+    return f
+
+
+def selector_mask(M, N, K, **kw):
+    m = np.empty(lib.hmlSelectorClassCount(), np.float32)
+    check(lib.hmlSelectorMask(ctypes.byref(_shape(M, N, K, **kw)), _fp(m)))
+    return m
+
+
+def selector_variant(M, N, K, cls, **kw):
+    v = SgemmVariant()
+    st = lib.hmlSelectorVariant(ctypes.byref(_shape(M, N, K, **kw)), cls, ctypes.byref(v))
+    return v if st == 0 else None
+
+
+def selector_class_of(M, N, K, variant, **kw):
+    v = variant if isinstance(variant, SgemmVariant) else SgemmVariant(*variant)
+    return lib.hmlSelectorClassOf(ctypes.byref(_shape(M, N, K, **kw)), ctypes.byref(v))
+
+
+def selector_install(net):
+    check(lib.hmlSelectorInstall(net.ptr if net is not None else None))
+
+
 def sgemm_select(M, N, K, order="row", transA=False, transB=False, beta=0.0):
     v = SgemmVariant()
     src = ctypes.c_int32()

@@ -84,6 +84,22 @@ _SIGS = {
     "hmlSgemmSelect": (c_i32, [c_i32, c_i32, c_i32, c_i64, c_i64, c_i64,
                                ctypes.c_float, ctypes.POINTER(SgemmVariant),
                                ctypes.POINTER(c_i32)]),
+    "hmlSgemmDense": (c_i32, [c_i32, c_i32, c_i32, c_i32, c_i64, c_i64, c_i64,
+                              ctypes.c_float, c_void_p, c_i64, c_void_p, c_i64,
+                              ctypes.c_float, c_void_p, c_i64, c_u32, c_void_p,
+                              c_size, c_void_p]),
+    "hmlSgemmDenseWorkspaceSize": (c_size, [c_i32, c_i64, c_i64, c_i64]),
+    "hmlSgemmGroupedCreate": (c_i32, [ctypes.POINTER(c_void_p), c_i32,
+                                      ctypes.POINTER(c_i64), ctypes.POINTER(c_i64),
+                                      ctypes.POINTER(c_i64), ctypes.POINTER(c_void_p),
+                                      ctypes.POINTER(c_i64), ctypes.POINTER(c_void_p),
+                                      ctypes.POINTER(c_i64), ctypes.POINTER(c_void_p),
+                                      ctypes.POINTER(c_i64), ctypes.c_float,
+                                      ctypes.c_float]),
+    "hmlSgemmGroupedRun": (c_i32, [c_void_p, c_i32, c_void_p]),
+    "hmlSgemmGroupedInfo": (c_i32, [c_void_p, ctypes.POINTER(c_i64),
+                                    ctypes.POINTER(c_i32), ctypes.POINTER(c_size)]),
+    "hmlSgemmGroupedDestroy": (c_i32, [c_void_p]),
     "hmlSgemvEx": (c_i32, [c_i32, c_i32, c_i64, c_i64, ctypes.c_float,
                            c_void_p, c_i64, c_void_p, c_i64, ctypes.c_float,
                            c_void_p, c_i64, c_u32, c_void_p, c_size,
@@ -127,6 +143,28 @@ _SIGS = {
                                       ctypes.POINTER(SgemmVariant)]),
     "hmlSgemvConfigAddRule": (c_i32, [c_i32, c_i32, c_i64, c_i64,
                                       ctypes.POINTER(SgemvVariant)]),
+    "hmlMlpCreate": (c_i32, [ctypes.POINTER(c_void_p), c_i32, c_i32, c_i32, c_i32,
+                             c_u64]),
+    "hmlMlpDestroy": (c_i32, [c_void_p]),
+    "hmlMlpParamCount": (c_i64, [c_void_p]),
+    "hmlMlpGetParams": (c_i32, [c_void_p, c_float_p, c_i64]),
+    "hmlMlpSetParams": (c_i32, [c_void_p, c_float_p, c_i64]),
+    "hmlMlpLossGrad": (c_i32, [c_void_p, c_float_p, c_float_p, c_float_p, c_i32,
+                               c_float_p, c_float_p]),
+    "hmlMlpTrain": (c_i32, [c_void_p, c_float_p, c_float_p, c_float_p, c_i64, c_i32,
+                            c_i32, ctypes.c_float, c_u64, c_float_p]),
+    "hmlMlpPredict": (c_i32, [c_void_p, c_float_p, c_float_p, c_i64, c_float_p]),
+    "hmlMlpSave": (c_i32, [c_void_p, ctypes.c_char_p]),
+    "hmlMlpLoad": (c_i32, [ctypes.POINTER(c_void_p), ctypes.c_char_p]),
+    "hmlSelectorFeatureCount": (c_i32, []),
+    "hmlSelectorClassCount": (c_i32, []),
+    "hmlSelectorFeatures": (c_i32, [ctypes.POINTER(GemmShape), c_float_p]),
+    "hmlSelectorMask": (c_i32, [ctypes.POINTER(GemmShape), c_float_p]),
+    "hmlSelectorVariant": (c_i32, [ctypes.POINTER(GemmShape), c_i32,
+                                   ctypes.POINTER(SgemmVariant)]),
+    "hmlSelectorClassOf": (c_i32, [ctypes.POINTER(GemmShape),
+                                   ctypes.POINTER(SgemmVariant)]),
+    "hmlSelectorInstall": (c_i32, [c_void_p]),
     "hmlFillUniform": (c_i32, [c_void_p, c_size, c_u64, ctypes.c_float,
                                ctypes.c_float, c_void_p]),
     "hmlSgemmCheckF64": (c_i32, [c_i32, c_i32, c_i32, c_i64, c_i64, c_i64,
@@ -135,6 +173,7 @@ _SIGS = {
                                  c_i64, c_double_p]),
     "hmlInit": (c_i32, []),
     "hmlDeviceCount": (ctypes.c_int, []),
+    "hmlSetDevice": (c_i32, [ctypes.c_int]),
     "hmlDeviceInfo": (c_i32, [ctypes.c_int, ctypes.c_char_p,
                               ctypes.POINTER(c_i32), ctypes.POINTER(c_i64),
                               ctypes.POINTER(c_i64), ctypes.POINTER(c_i32)]),

@@ -32,6 +32,7 @@
 #include "hml_config.h"
 #include "hml_internal.h"
 #include "hml_runtime.h"
+#include "hml_mlp.h"
 #include "hml_select.h"
 
 namespace hml {
@@ -146,26 +147,13 @@ std::vector<hmlSgemmVariant> sgemm_candidates(const GemmKey &k) {
     for (const hmlSgemvVariant &g : sgemv_candidates(gk))
       push_unique(&out, hmlSgemmVariant{HML_SGEMM_FAMILY_GEMV, g.lpr, g.rb, g.split, g.wpb});
   }
-  const int64_t nct = ceil_div(k.N, 16);
-  int ctmax = 1;
-  while (ctmax < 8 && ctmax < nct) ctmax <<= 1;
-  const int family = k.form == kFormDot ? HML_SGEMM_FAMILY_DOT : HML_SGEMM_FAMILY_AXPY;
-  const std::vector<int> rts = k.form == kFormDot ? std::vector<int>{1, 2, 4} : std::vector<int>{1, 2};
-  for (int ct : {ctmax, ctmax / 2}) {
-    if (ct < 1) continue;
-    for (int rt : rts) {
-      if (rt * ct > 16) continue;  // register budget
-      const int64_t rows = (k.form == kFormDot ? 16 : 64) * rt;
-      const int64_t blocks = ceil_div(k.M, rows) * ceil_div(nct, ct);
-      int64_t heur = blocks >= 4096 ? 1 : ceil_div(4096, blocks);
-      const int64_t cap = std::max<int64_t>(1, std::min<int64_t>(64, k.K / 128));
-      heur = std::min(heur, cap);
-      for (int ks : split_ladder(heur, cap))
-        for (int wpb : {2, 4}) {
-          hmlSgemmVariant v{family, rt, ct, ks, wpb};
-          if (sgemm_variant_runnable(k, v)) push_unique(&out, v);
-        }
-    }
+  // Every skinny / dense class of the learned selector that can run the
+  // shape (row tiles 1/2/4, full or half column tiles, K split 1..64 up to
+  // K/128, 2 or 4 waves per block), so the timing database labels exactly
+  // the choices the selector can make.
+  for (int c = 0; c < kSelClasses; ++c) {
+    hmlSgemmVariant v;
+    if (selector_variant(k, c, &v)) push_unique(&out, v);
   }
   if (sgemm_variant_runnable(k, h)) push_unique(&out, h);
   return out;

@@ -96,6 +96,8 @@ GemmKey make_key(const GemmCanon &g, float beta, uint32_t flags, hmlOrder order,
   k.beta0 = (beta == 0.0f && !(flags & HML_FLAG_READ_C_ALWAYS)) ? 1 : 0;
   k.transA = ta; k.transB = tb; k.order = order;
   k.gemv_ok = g.N == 1 && (g.sBk == 1 || g.K == 1) && (g.sCm == 1 || g.M == 1);
+  k.cM = g.swapped ? g.N : g.M;
+  k.cN = g.swapped ? g.M : g.N;
   return k;
 }
 
@@ -164,6 +166,11 @@ void heuristic_sgemm(const GemmKey &k, hmlSgemmVariant *v) {
     v->rt = gv.lpr; v->ct = gv.rb; v->ksplit = gv.split; v->wpb = gv.wpb;
     return;
   }
+  if (k.N > 128 && k.M >= 256 && k.K >= 64) {
+    // Not skinny: plain FP32 dense tiles (A re-read N/128 times otherwise).
+    *v = hmlSgemmVariant{HML_SGEMM_FAMILY_DENSE, 1, 1, 1, 4};
+    return;
+  }
   const int64_t nct = ceil_div(k.N, 16);
   v->ct = nct <= 1 ? 1 : nct <= 2 ? 2 : nct <= 4 ? 4 : 8;
   v->wpb = 4;
@@ -203,6 +210,7 @@ bool sgemm_variant_runnable(const GemmKey &k, const hmlSgemmVariant &v) {
     hmlSgemvVariant gv{gk.family, v.rt, v.ct, v.ksplit, v.wpb};
     return sgemv_variant_runnable(gk, gv);
   }
+  if (is_dense_family(v.family)) return true;
   if (v.ksplit < 1 || v.ksplit > 4096) return false;
   if (v.family == HML_SGEMM_FAMILY_DOT) return k.form == kFormDot && skinny_has_variant(kFormDot, v.rt, v.ct);
   if (v.family == HML_SGEMM_FAMILY_AXPY) return k.form == kFormAxpy && skinny_has_variant(kFormAxpy, v.rt, v.ct);
@@ -210,6 +218,10 @@ bool sgemm_variant_runnable(const GemmKey &k, const hmlSgemmVariant &v) {
 }
 
 void select_sgemm(const GemmKey &k, hmlSgemmVariant *v, int *source) {
+  if (learned_select_sgemm(k, v) && sgemm_variant_runnable(k, *v)) {
+    *source = 2;
+    return;
+  }
   if (config_select_sgemm(k, v, source) && sgemm_variant_runnable(k, *v)) return;
   heuristic_sgemm(k, v);
   *source = 0;
@@ -220,6 +232,8 @@ void select_sgemm(const GemmKey &k, hmlSgemmVariant *v, int *source) {
 namespace {
 
 size_t workspace_bytes(const GemmKey &k, const hmlSgemmVariant &v) {
+  if (is_dense_family(v.family))
+    return dense_workspace(dense_precision(v.family), k.M, k.N, k.K);
   if (v.family == HML_SGEMM_FAMILY_GEMV) {
     return v.ksplit > 1 ? align256((size_t)v.ksplit * k.M * sizeof(float)) : 0;
   }
@@ -245,6 +259,10 @@ hmlStatus run_sgemm(const GemmCanon &g, const GemmKey &k,
   }
   if (v.family == HML_SGEMM_FAMILY_GEMV)
     return run_gemv_family(g, k, v, alpha, beta, read_c, ws, st);
+  if (is_dense_family(v.family))
+    return run_dense(dense_precision(v.family), g.A, g.sAm, g.sAk, g.B, g.sBk,
+                     g.sBn, g.C, g.sCm, g.sCn, g.M, g.N, g.K, alpha, beta,
+                     read_c, ws, ws ? need : 0, st);
 
   const SkinnyPlan pl = plan_skinny(k, v);
   float *Bp = (float *)ws;
@@ -266,7 +284,8 @@ hmlStatus run_sgemm(const GemmCanon &g, const GemmKey &k,
   if (k.form == kFormDot)
     vec = ((uintptr_t)g.A % 16 == 0) && (g.sAm % 4 == 0 || g.M == 1);
   else
-    vec = ((uintptr_t)g.A % 16 == 0) && (g.sAk % 4 == 0 || g.K == 1);
+    vec = ((uintptr_t)g.A % 16 == 0) && (g.sAk % 4 == 0 || g.K == 1) &&
+          g.M % 4 == 0;
   e = launch_skinny(k.form, v.rt, v.ct, vec, v.wpb, p, st);
   if (e != hipSuccess) return from_hip(e, "skinny kernel launch");
   if (pl.ksplit > 1) {
@@ -291,7 +310,7 @@ hmlStatus run_gemv_family(const GemmCanon &g, const GemmKey &k,
   chunk = ceil_div(chunk, gran) * gran;
   p.chunk = chunk;
   p.split = (int)ceil_div(std::max<int64_t>(reduced, 1), chunk);
-  bool vec = ((uintptr_t)p.A % 16 == 0) && (p.lda % 4 == 0) &&
+  bool vec = ((uintptr_t)p.A % 16 == 0) && (p.lda % 4 == 0) && (p.N % 4 == 0) &&
              (family != HML_SGEMV_FAMILY_DOT || (uintptr_t)p.x % 16 == 0);
   hipError_t e = launch_gemv(family, v.rt, v.ct, vec, v.wpb, p, st);
   if (e != hipSuccess) return from_hip(e, "gemv kernel launch");

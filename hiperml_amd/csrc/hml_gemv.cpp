@@ -179,7 +179,9 @@ hmlStatus sgemv_impl(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
     }
   }
   p.P = (float *)ws;
-  const bool vec = ((uintptr_t)A % 16 == 0) && (lda % 4 == 0) &&
+  // The streaming kernels need whole 4-float groups along the contiguous
+  // extent (p.N in both forms) and 16-byte aligned rows.
+  const bool vec = ((uintptr_t)A % 16 == 0) && (lda % 4 == 0) && (p.N % 4 == 0) &&
                    (c.family != HML_SGEMV_FAMILY_DOT || (uintptr_t)x % 16 == 0);
   hipError_t e = launch_gemv(v.family, v.lpr, v.rb, vec, v.wpb, p, st);
   if (e != hipSuccess) return from_hip(e, "gemv kernel launch");

@@ -1,0 +1,218 @@
+// hml_grouped.cpp -- grouped (batched) skinny SGEMM: a plan holds the device
+// descriptors of many independent row-major problems, their packed B, and a
+// longest-first tile schedule; running it is two launches (batched B pack +
+// persistent grouped kernel), optionally replayed from a captured hipGraph.
+// No reference counterpart (the reference runs one problem per call,
+// src/hml_sgemm.cu:120-211).
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "hml_internal.h"
+#include "hml_runtime.h"
+#include "hml_select.h"
+
+struct hmlGroupedPlan_t {
+  int32_t count = 0;
+  int64_t ntiles = 0;
+  int64_t max_pack = 0;
+  int nblocks = 0;
+  void *dev = nullptr;  // one allocation: problems | packs | tiles | Bp
+  hml::GroupProblem *probs = nullptr;
+  hml::GroupPack *packs = nullptr;
+  hml::GroupTile *tiles = nullptr;
+  size_t bytes = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t graph_stream = nullptr;
+};
+
+namespace hml {
+namespace {
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+hmlStatus record(hmlGroupedPlan_t *p, hipStream_t st) {
+  hipError_t e = launch_pack_group(p->packs, p->count, p->max_pack, st);
+  if (e != hipSuccess) return from_hip(e, "grouped pack");
+  e = launch_grouped(p->probs, p->tiles, p->ntiles, p->nblocks, st);
+  return from_hip(e, "grouped kernel");
+}
+
+}  // namespace
+}  // namespace hml
+
+using namespace hml;
+
+extern "C" {
+
+hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan_t **plan, int32_t count,
+                                const int64_t *M, const int64_t *N,
+                                const int64_t *K, const float *const *A,
+                                const int64_t *lda, const float *const *B,
+                                const int64_t *ldb, float *const *C,
+                                const int64_t *ldc, float alpha, float beta) {
+  clear_error();
+  if (!plan || count <= 0 || !M || !N || !K || !A || !lda || !B || !ldb || !C || !ldc)
+    return fail(HML_STATUS_INVALID_VALUE, "grouped: NULL argument or count <= 0");
+  if (count > 65535) return fail(HML_STATUS_INVALID_VALUE, "grouped: at most 65535 problems");
+  std::vector<GroupProblem> probs(count);
+  std::vector<GroupPack> packs(count);
+  std::vector<size_t> bp_off(count);
+  std::vector<int64_t> tile_cost;
+  std::vector<GroupTile> tiles;
+  size_t bp_total = 0;
+  int64_t max_pack = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    if (M[i] <= 0 || N[i] <= 0 || K[i] <= 0)
+      return fail(HML_STATUS_INVALID_VALUE, "grouped: problem %d has a zero extent", i);
+    if (lda[i] < K[i] || ldb[i] < N[i] || ldc[i] < N[i])
+      return fail(HML_STATUS_INVALID_VALUE, "grouped: problem %d leading dimension too small", i);
+    if (!A[i] || !B[i] || !C[i]) return fail(HML_STATUS_INVALID_VALUE, "grouped: problem %d NULL", i);
+    // Column tiles per pass from the selector (config store / heuristic),
+    // limited to the grouped kernel's instantiations.
+    GemmCanon g = canonicalize(HML_ROW_MAJOR, HML_OP_N, HML_OP_N, M[i], N[i], K[i],
+                               A[i], lda[i], B[i], ldb[i], C[i], ldc[i]);
+    if (g.swapped || g.form != kFormDot)
+      return fail(HML_STATUS_UNSUPPORTED,
+                  "grouped: problem %d has N > M (only tall problems are grouped)", i);
+    GemmKey key = make_key(g, beta, 0, HML_ROW_MAJOR, HML_OP_N, HML_OP_N);
+    key.gemv_ok = false;
+    hmlSgemmVariant v;
+    int src;
+    select_sgemm(key, &v, &src);
+    const int64_t nct = ceil_div(N[i], 16);
+    int ct = (v.family == HML_SGEMM_FAMILY_DOT) ? v.ct : 4;
+    while (ct > 1 && ct / 2 >= nct) ct /= 2;
+    if (ct > 4) ct = 4;
+    const int nct_pad = (int)(ceil_div(nct, ct) * ct);
+    GroupProblem &p = probs[i];
+    p.A = A[i]; p.lda = lda[i]; p.C = C[i]; p.ldc = ldc[i];
+    p.M = M[i]; p.N = N[i]; p.K = K[i];
+    p.nct_pad = nct_pad; p.ct = ct;
+    p.vec = ((uintptr_t)A[i] % 16 == 0 && lda[i] % 4 == 0) ? 1 : 0;
+    p.read_c = beta != 0.0f ? 1 : 0;
+    p.alpha = alpha; p.beta = beta;
+    const int64_t nsteps = ceil_div(K[i], kDotKStep);
+    const int64_t elems = nsteps * nct_pad * 128;  // f4 fragments
+    bp_off[i] = bp_total;
+    bp_total += align256((size_t)elems * 16);
+    max_pack = std::max(max_pack, elems);
+    GroupPack &q = packs[i];
+    q.B = B[i]; q.sBk = ldb[i]; q.sBn = 1; q.K = K[i]; q.N = N[i];
+    q.nct_pad = nct_pad; q.pad = 0;
+    const int64_t nrb = ceil_div(M[i], kDotRowsPerTile);
+    // cost of one 16-row tile: A bytes plus C bytes, in 4-byte words
+    const int64_t cost = 16 * K[i] + 16 * N[i] * (1 + p.read_c) + 64;
+    for (int64_t rb = 0; rb < nrb; ++rb) {
+      tiles.push_back(GroupTile{i, 0, rb});
+      tile_cost.push_back(cost);
+    }
+  }
+  // longest-first order (stable: equal costs keep problem order)
+  std::vector<int64_t> order(tiles.size());
+  for (size_t t = 0; t < order.size(); ++t) order[t] = (int64_t)t;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t a, int64_t b) { return tile_cost[a] > tile_cost[b]; });
+  std::vector<GroupTile> sorted(tiles.size());
+  for (size_t t = 0; t < order.size(); ++t) sorted[t] = tiles[order[t]];
+
+  hmlGroupedPlan_t *pl = new hmlGroupedPlan_t();
+  pl->count = count;
+  pl->ntiles = (int64_t)sorted.size();
+  pl->max_pack = max_pack;
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+  }
+  // persistent grid: 2 blocks of 4 waves per CU, never more blocks than
+  // needed to give every wave a tile
+  pl->nblocks = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)cus * 2, ceil_div(pl->ntiles, 4)));
+  const size_t o_probs = 0;
+  const size_t o_packs = align256(o_probs + sizeof(GroupProblem) * count);
+  const size_t o_tiles = align256(o_packs + sizeof(GroupPack) * count);
+  const size_t o_bp = align256(o_tiles + sizeof(GroupTile) * sorted.size());
+  pl->bytes = o_bp + bp_total;
+  hipError_t e = hipMalloc(&pl->dev, pl->bytes);
+  if (e != hipSuccess) {
+    delete pl;
+    return fail(HML_STATUS_ALLOC_FAILED, "grouped plan hipMalloc(%zu): %s",
+                o_bp + bp_total, hipGetErrorString(e));
+  }
+  char *base = (char *)pl->dev;
+  for (int32_t i = 0; i < count; ++i) {
+    probs[i].Bp = (const float *)(base + o_bp + bp_off[i]);
+    packs[i].Bp = (float *)(base + o_bp + bp_off[i]);
+  }
+  pl->probs = (GroupProblem *)(base + o_probs);
+  pl->packs = (GroupPack *)(base + o_packs);
+  pl->tiles = (GroupTile *)(base + o_tiles);
+  e = hipMemcpy(pl->probs, probs.data(), sizeof(GroupProblem) * count, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(pl->packs, packs.data(), sizeof(GroupPack) * count, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(pl->tiles, sorted.data(), sizeof(GroupTile) * sorted.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(pl->dev);
+    delete pl;
+    return from_hip(e, "grouped plan upload");
+  }
+  *plan = pl;
+  return HML_STATUS_SUCCESS;
+}
+
+hmlStatus hmlSgemmGroupedRun(hmlGroupedPlan_t *plan, int32_t use_graph,
+                             void *stream) {
+  clear_error();
+  if (!plan) return fail(HML_STATUS_INVALID_VALUE, "plan is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  if (!use_graph) return record(plan, st);
+  if (!plan->exec) {
+    // Capture on a private stream (the null stream cannot be captured).
+    hipError_t e = hipStreamCreate(&plan->graph_stream);
+    if (e != hipSuccess) return from_hip(e, "hipStreamCreate");
+    e = hipStreamBeginCapture(plan->graph_stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return from_hip(e, "hipStreamBeginCapture");
+    hmlStatus s = record(plan, plan->graph_stream);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(plan->graph_stream, &g);
+    if (s != HML_STATUS_SUCCESS) {
+      if (g) hipGraphDestroy(g);
+      return s;
+    }
+    if (e != hipSuccess) return from_hip(e, "hipStreamEndCapture");
+    plan->graph = g;
+    e = hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      plan->exec = nullptr;
+      return from_hip(e, "hipGraphInstantiate");
+    }
+  }
+  return from_hip(hipGraphLaunch(plan->exec, st), "hipGraphLaunch");
+}
+
+hmlStatus hmlSgemmGroupedInfo(hmlGroupedPlan_t *plan, int64_t *ntiles,
+                              int32_t *nblocks, size_t *device_bytes) {
+  if (!plan) return fail(HML_STATUS_INVALID_VALUE, "plan is NULL");
+  if (ntiles) *ntiles = plan->ntiles;
+  if (nblocks) *nblocks = plan->nblocks;
+  if (device_bytes) *device_bytes = plan->bytes;
+  return HML_STATUS_SUCCESS;
+}
+
+hmlStatus hmlSgemmGroupedDestroy(hmlGroupedPlan_t *plan) {
+  if (!plan) return HML_STATUS_SUCCESS;
+  hipDeviceSynchronize();
+  if (plan->exec) hipGraphExecDestroy(plan->exec);
+  if (plan->graph) hipGraphDestroy(plan->graph);
+  if (plan->graph_stream) hipStreamDestroy(plan->graph_stream);
+  if (plan->dev) hipFree(plan->dev);
+  delete plan;
+  return HML_STATUS_SUCCESS;
+}
+
+}  // extern "C"

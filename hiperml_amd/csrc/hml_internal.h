@@ -102,3 +102,39 @@ hipError_t launch_compare(const float *C, const float *R, int64_t M, int64_t N,
                           int64_t sCm, int64_t sCn, double *stats, int nblocks,
                           hipStream_t st);
 }  // namespace hml
+
+namespace hml {
+// One problem of a grouped (batched) skinny SGEMM, row-major NN after
+// canonicalisation (DOT form): C(m,n) = alpha sum_k A[m*lda+k] B(k,n) +
+// beta C[m*ldc+n], B already packed into Bp.
+struct GroupProblem {
+  const float *A;
+  int64_t lda;
+  const float *Bp;
+  float *C;
+  int64_t ldc;
+  int64_t M, N, K;
+  int32_t nct_pad;  // column tiles in Bp (multiple of ct)
+  int32_t ct;       // column tiles per pass: 1, 2 or 4
+  int32_t vec;      // A 16-byte aligned with lda % 4 == 0
+  int32_t read_c;
+  float alpha, beta;
+};
+struct GroupTile {
+  int32_t prob;
+  int32_t pad;
+  int64_t rb;  // row block of 16 rows
+};
+struct GroupPack {
+  const float *B;
+  int64_t sBk, sBn, K, N;
+  int32_t nct_pad;
+  int32_t pad;
+  float *Bp;
+};
+hipError_t launch_grouped(const GroupProblem *probs, const GroupTile *tiles,
+                          int64_t ntiles, int nblocks, hipStream_t st);
+// Packs every problem's B (DOT form) in one launch.
+hipError_t launch_pack_group(const GroupPack *packs, int32_t count,
+                             int64_t max_elems, hipStream_t st);
+}  // namespace hml

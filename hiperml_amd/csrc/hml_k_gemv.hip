@@ -179,6 +179,235 @@ __global__ __launch_bounds__(256) void k_gemv_axpy(GemvParams p) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Streaming variants used when A, x are 16-byte aligned, lda % 4 == 0 and the
+// contiguous extent is a multiple of 4: every lane's 4-float group is then
+// wholly in range or wholly out.  Whole steps go through a guard-free
+// pipeline with P steps of loads in flight in P named register buffers (the
+// loop is unrolled by P, so nothing is copied and only the oldest step is
+// waited for); A is loaded non-temporally (read once); the last partial
+// step uses clamped addresses with zeroed values.
+// ---------------------------------------------------------------------------
+template <int LPR, int RB>
+__global__ __launch_bounds__(256) void k_gemv_dot_stream(GemvParams p) {
+  constexpr int G = 64 / LPR;
+  constexpr int P = RB <= 2 ? 4 : 2;
+  constexpr int STEP = 4 * LPR;
+  const int lane = threadIdx.x & 63;
+  const int wpb = blockDim.x >> 6;
+  const int64_t task = (int64_t)blockIdx.x * wpb + (threadIdx.x >> 6);
+  const int64_t rows_per_wave = (int64_t)G * RB;
+  const int64_t nrb = (p.M + rows_per_wave - 1) / rows_per_wave;
+  const int64_t rb = task / p.split;
+  const int sp = (int)(task - rb * p.split);
+  if (rb >= nrb) return;  // no barriers in this kernel
+  const int grp = lane / LPR, q = lane % LPR;
+
+  const int64_t cb = (int64_t)sp * p.chunk;
+  int64_t ce = cb + p.chunk;
+  if (ce > p.N) ce = p.N;
+  const int64_t nfull = ce > cb ? (ce - cb) / STEP : 0;
+
+  const float *ap[RB];
+  int64_t row[RB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i) {
+    row[i] = rb * rows_per_wave + (int64_t)i * G + grp;
+    const int64_t rc = row[i] < p.M ? row[i] : p.M - 1;
+    ap[i] = p.A + rc * p.lda + cb + 4 * q;
+  }
+  const float *xp = p.x + cb + 4 * q;
+
+  float acc[RB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i) acc[i] = 0.f;
+
+  f4 bx[P], ba[P][RB];
+  auto load = [&](int j, int64_t s) {
+    bx[j] = *reinterpret_cast<const f4 *>(xp + s * STEP);
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+      ba[j][i] = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(ap[i] + s * STEP));
+  };
+  auto comp = [&](int j) {
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      acc[i] += ba[j][i][0] * bx[j][0];
+      acc[i] += ba[j][i][1] * bx[j][1];
+      acc[i] += ba[j][i][2] * bx[j][2];
+      acc[i] += ba[j][i][3] * bx[j][3];
+    }
+  };
+  if (nfull >= P) {
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      load(j, j);
+      __builtin_amdgcn_sched_barrier(0);  // issue the stages in order
+    }
+    int64_t s = 0;
+    for (; s + 2 * P <= nfull; s += P) {
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        comp(j);
+        load(j, s + P + j);
+        __builtin_amdgcn_sched_barrier(0);  // keep each stage's loads right after its compute
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      comp(j);
+      if (s + P + j < nfull) load(j, s + P + j);
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j)
+      if (s + P + j < nfull) comp(j);
+  } else {
+    for (int64_t s = 0; s < nfull; ++s) {
+      load(0, s);
+      comp(0);
+    }
+  }
+  {
+    // last partial step: lanes past ce read column cb (in range) and zero it
+    const int64_t c = cb + nfull * STEP + 4 * q;
+    if (cb + nfull * STEP < ce) {
+      const bool in = c < ce;
+      const int64_t cc = in ? c : cb;
+      f4 xv = *reinterpret_cast<const f4 *>(p.x + cc);
+      if (!in) xv = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int i = 0; i < RB; ++i) {
+        const int64_t rc = row[i] < p.M ? row[i] : p.M - 1;
+        f4 av = *reinterpret_cast<const f4 *>(p.A + rc * p.lda + cc);
+        if (!in) av = f4{0.f, 0.f, 0.f, 0.f};
+        acc[i] += av[0] * xv[0];
+        acc[i] += av[1] * xv[1];
+        acc[i] += av[2] * xv[2];
+        acc[i] += av[3] * xv[3];
+      }
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < RB; ++i) {
+    const float s = group_sum<LPR>(acc[i]);
+    if (q == 0 && row[i] < p.M) {
+      if (p.split == 1) {
+        float *yp = p.y + row[i];
+        float v = p.alpha * s;
+        if (p.read_c) v = p.alpha * s + p.beta * (*yp);
+        *yp = v;
+      } else {
+        p.P[(int64_t)sp * p.M + row[i]] = s;
+      }
+    }
+  }
+}
+
+template <int LPR>
+__global__ __launch_bounds__(256) void k_gemv_axpy_stream(GemvParams p) {
+  constexpr int G = 64 / LPR;
+  constexpr int CW = 4 * LPR;
+  constexpr int P = 4;
+  __shared__ f4 red[4][LPR];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int wpb = blockDim.x >> 6;
+  const int grp = lane / LPR, q = lane % LPR;
+  const int64_t c0 = (int64_t)blockIdx.x * CW + 4 * q;
+  const bool cin = c0 < p.N;          // N % 4 == 0: whole group in or out
+  const int64_t cc = cin ? c0 : 0;
+  const int sp = blockIdx.y;
+  const int64_t rb = (int64_t)sp * p.chunk;
+  int64_t re = rb + p.chunk;
+  if (re > p.M) re = p.M;
+  const int64_t rstep = (int64_t)wpb * G;
+  const int64_t r0 = rb + (int64_t)w * G + grp;
+  // whole steps: every lane group of the wave has its row inside [rb, re)
+  const int64_t first_last = rb + (int64_t)w * G + (G - 1);
+  const int64_t nfull = re > first_last ? (re - first_last - 1) / rstep + 1 : 0;
+
+  f4 acc = {0.f, 0.f, 0.f, 0.f};
+  const float *ap = p.A + r0 * p.lda + cc;
+  const float *xp = p.x + r0;
+  const int64_t astep = rstep * p.lda;
+  f4 ba[P];
+  float bx[P];
+  auto load = [&](int j, int64_t s) {
+    bx[j] = xp[s * rstep];
+    ba[j] = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(ap + s * astep));
+  };
+  auto comp = [&](int j) { acc += ba[j] * bx[j]; };
+  if (nfull >= P) {
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      load(j, j);
+      __builtin_amdgcn_sched_barrier(0);  // issue the stages in order
+    }
+    int64_t s = 0;
+    for (; s + 2 * P <= nfull; s += P) {
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        comp(j);
+        load(j, s + P + j);
+        __builtin_amdgcn_sched_barrier(0);  // keep each stage's loads right after its compute
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      comp(j);
+      if (s + P + j < nfull) load(j, s + P + j);
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j)
+      if (s + P + j < nfull) comp(j);
+  } else {
+    for (int64_t s = 0; s < nfull; ++s) {
+      load(0, s);
+      comp(0);
+    }
+  }
+  {
+    // rows after the whole steps (at most one partial step per lane group)
+    const int64_t r = r0 + nfull * rstep;
+    if (r < re) {
+      const float xr = p.x[r];
+      const f4 av = *reinterpret_cast<const f4 *>(p.A + r * p.lda + cc);
+      acc += av * xr;
+    }
+  }
+  if (!cin) acc = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int off = LPR; off < 64; off <<= 1) {
+    acc[0] += __shfl_xor(acc[0], off);
+    acc[1] += __shfl_xor(acc[1], off);
+    acc[2] += __shfl_xor(acc[2], off);
+    acc[3] += __shfl_xor(acc[3], off);
+  }
+  if (wpb > 1) {
+    if (grp == 0) red[w][q] = acc;
+    __syncthreads();
+    if (w == 0 && grp == 0) {
+      acc = red[0][q];
+      for (int i = 1; i < wpb; ++i) acc += red[i][q];
+    }
+  }
+  if (w == 0 && grp == 0 && cin) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int64_t col = c0 + t;
+      if (p.split == 1) {
+        float *yp = p.y + col;
+        float v = p.alpha * acc[t];
+        if (p.read_c) v = p.alpha * acc[t] + p.beta * (*yp);
+        *yp = v;
+      } else {
+        p.P[(int64_t)sp * p.N + col] = acc[t];
+      }
+    }
+  }
+}
+
 typedef hipError_t (*RunFn)(int, const GemvParams &, hipStream_t);
 
 template <int LPR, int RB, bool VEC>
@@ -187,8 +416,12 @@ hipError_t run_dot(int wpb, const GemvParams &p, hipStream_t st) {
   const int64_t tasks = ((p.M + rows - 1) / rows) * p.split;
   const int64_t nblk = (tasks + wpb - 1) / wpb;
   if (nblk > 0x7fffffff) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL((k_gemv_dot<LPR, RB, VEC>), dim3((unsigned)nblk),
-                     dim3(64 * wpb), 0, st, p);
+  if (VEC)
+    hipLaunchKernelGGL((k_gemv_dot_stream<LPR, RB>), dim3((unsigned)nblk),
+                       dim3(64 * wpb), 0, st, p);
+  else
+    hipLaunchKernelGGL((k_gemv_dot<LPR, RB, false>), dim3((unsigned)nblk),
+                       dim3(64 * wpb), 0, st, p);
   return hipGetLastError();
 }
 
@@ -196,8 +429,12 @@ template <int LPR, bool VEC>
 hipError_t run_axpy(int wpb, const GemvParams &p, hipStream_t st) {
   const int64_t ncb = (p.N + 4 * LPR - 1) / (4 * LPR);
   if (ncb > 0x7fffffff || p.split > 65535) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL((k_gemv_axpy<LPR, VEC>), dim3((unsigned)ncb, (unsigned)p.split),
-                     dim3(64 * wpb), 0, st, p);
+  if (VEC)
+    hipLaunchKernelGGL((k_gemv_axpy_stream<LPR>), dim3((unsigned)ncb, (unsigned)p.split),
+                       dim3(64 * wpb), 0, st, p);
+  else
+    hipLaunchKernelGGL((k_gemv_axpy<LPR, false>), dim3((unsigned)ncb, (unsigned)p.split),
+                       dim3(64 * wpb), 0, st, p);
   return hipGetLastError();
 }
 

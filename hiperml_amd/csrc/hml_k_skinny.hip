@@ -51,6 +51,19 @@ __device__ __forceinline__ f4 load4(const float *p, int64_t nvalid) {
   return v;
 }
 
+// Unguarded load of 4 consecutive floats that are read exactly once
+// (non-temporal, so the streamed operand does not evict B from the caches).
+template <bool VEC>
+__device__ __forceinline__ f4 stream4(const float *p) {
+  if (VEC) return __builtin_nontemporal_load(reinterpret_cast<const f4 *>(p));
+  f4 v;
+  v[0] = __builtin_nontemporal_load(p);
+  v[1] = __builtin_nontemporal_load(p + 1);
+  v[2] = __builtin_nontemporal_load(p + 2);
+  v[3] = __builtin_nontemporal_load(p + 3);
+  return v;
+}
+
 __device__ __forceinline__ void store_out(const SkinnyParams &p, int64_t row,
                                           int64_t col, int ks, float acc) {
   if (row < p.M && col < p.N) {
@@ -65,25 +78,30 @@ __device__ __forceinline__ void store_out(const SkinnyParams &p, int64_t row,
   }
 }
 
+// Both kernels stream K in steps.  Steps that lie wholly inside the wave's
+// K range go through a guard-free pipeline that keeps P steps of loads in
+// flight in P named register buffers (the loop is unrolled by P, so no
+// buffer is ever copied and the compiler waits only for the oldest step);
+// a partial last step (K not a multiple of the step) is loaded with bounds
+// checks afterwards.
+
 // ---------------------------------------------------------------------------
 // DOT form: A(m,k) = A[m*sAm + k].  Per k-step of 32, lane (g = l/16,
-// r = l%16) of row tile t loads A(row_t + r, k0 + 8g .. k0 + 8g + 7) as two
-// 128-bit loads; MFMA number 4v+j consumes element j of load v, so the
-// shared K permutation is k = k0 + 8g + 4v + j.  The packed B fragment of
+// r = l%16) of row tile t loads A(row_t + r, k0 + 16v + 4g .. +3) for
+// v = 0, 1: each 128-bit load instruction reads 64 contiguous bytes of each
+// of its 16 rows.  MFMA number 4v+j consumes element j of load v, so the
+// shared K permutation is k = k0 + 16v + 4g + j.  The packed B fragment of
 // (step s, column tile c, half v) holds, for lane (g, r),
-// B(32s + 8g + 4v + j, 16c + r) for j = 0..3.
+// B(32s + 16v + 4g + j, 16c + r) for j = 0..3.
 // ---------------------------------------------------------------------------
+// One wave computes rows [rb*16*RT, (rb+1)*16*RT) x column tiles
+// [ct0, ct0+CT) over the K range of split ks.
 template <int RT, int CT, bool VEC>
-__global__ __launch_bounds__(256) void k_skinny_dot(SkinnyParams p) {
+__device__ __forceinline__ void dot_tile(const SkinnyParams &p, int64_t rb,
+                                         int ks, int ct0) {
+  constexpr int P = (RT + CT <= 3) ? 4 : 2;
   const int lane = threadIdx.x & 63;
-  const int wpb = blockDim.x >> 6;
-  const int64_t task = (int64_t)blockIdx.x * wpb + (threadIdx.x >> 6);
   const int64_t rows_per_wave = (int64_t)kDotRowsPerTile * RT;
-  const int64_t nrb = (p.M + rows_per_wave - 1) / rows_per_wave;
-  const int64_t rb = task / p.ksplit;
-  const int ks = (int)(task - rb * p.ksplit);
-  if (rb >= nrb) return;  // whole wave leaves; the kernel has no barriers
-  const int ct0 = blockIdx.y * CT;
   const int r = lane & 15, g = lane >> 4;
 
   const float *arow[RT];
@@ -96,51 +114,89 @@ __global__ __launch_bounds__(256) void k_skinny_dot(SkinnyParams p) {
   const int64_t kb = (int64_t)ks * p.kchunk;
   int64_t ke = kb + p.kchunk;
   if (ke > p.K) ke = p.K;
+  const int64_t nfull = ke > kb ? (ke - kb) / kDotKStep : 0;
 
   const f4 *Bp = reinterpret_cast<const f4 *>(p.Bp);
+  const int64_t bstep = (int64_t)p.nct_pad * 128;  // f4 per k-step
+  const f4 *bp = Bp + (kb / kDotKStep) * bstep + (int64_t)ct0 * 128 + lane;
+  const float *ap[RT];
+#pragma unroll
+  for (int t = 0; t < RT; ++t) ap[t] = arow[t] + kb + 4 * g;
+
   f4 acc[RT][CT];
 #pragma unroll
   for (int t = 0; t < RT; ++t)
 #pragma unroll
     for (int c = 0; c < CT; ++c) acc[t][c] = f4{0.f, 0.f, 0.f, 0.f};
 
-  f4 a[RT][2], b[CT][2];
-  auto load_step = [&](int64_t k0, f4 (&aa)[RT][2], f4 (&bb)[CT][2]) {
-    const int64_t kk = k0 + 8 * g;
+  f4 a[P][RT][2], b[P][CT][2];
+  auto load = [&](int j, int64_t s) {
 #pragma unroll
     for (int t = 0; t < RT; ++t) {
-      aa[t][0] = load4<VEC>(arow[t] + kk, ke - kk);
-      aa[t][1] = load4<VEC>(arow[t] + kk + 4, ke - kk - 4);
+      a[j][t][0] = stream4<VEC>(ap[t] + s * kDotKStep);
+      a[j][t][1] = stream4<VEC>(ap[t] + s * kDotKStep + 16);
     }
-    const int64_t s = k0 / kDotKStep;
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
-      const int64_t base = ((s * p.nct_pad + ct0 + c) * 2) * 64 + lane;
-      bb[c][0] = Bp[base];
-      bb[c][1] = Bp[base + 64];
+      b[j][c][0] = bp[s * bstep + c * 128];
+      b[j][c][1] = bp[s * bstep + c * 128 + 64];
     }
   };
-
-  if (kb < ke) load_step(kb, a, b);
-  for (int64_t k0 = kb; k0 < ke; k0 += kDotKStep) {
-    f4 an[RT][2], bn[CT][2];
-    const bool more = k0 + kDotKStep < ke;
-    if (more) load_step(k0 + kDotKStep, an, bn);
+  auto comp = [&](int j) {
 #pragma unroll
     for (int v = 0; v < 2; ++v)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int jj = 0; jj < 4; ++jj)
 #pragma unroll
         for (int t = 0; t < RT; ++t)
 #pragma unroll
           for (int c = 0; c < CT; ++c)
-            acc[t][c] = mfma4(a[t][v][j], b[c][v][j], acc[t][c]);
-    if (more) {
+            acc[t][c] = mfma4(a[j][t][v][jj], b[j][c][v][jj], acc[t][c]);
+  };
+
+  if (nfull >= P) {
 #pragma unroll
-      for (int t = 0; t < RT; ++t) { a[t][0] = an[t][0]; a[t][1] = an[t][1]; }
-#pragma unroll
-      for (int c = 0; c < CT; ++c) { b[c][0] = bn[c][0]; b[c][1] = bn[c][1]; }
+    for (int j = 0; j < P; ++j) {
+      load(j, j);
+      __builtin_amdgcn_sched_barrier(0);  // issue the stages in order
     }
+    int64_t s = 0;
+    for (; s + 2 * P <= nfull; s += P) {
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        comp(j);
+        load(j, s + P + j);
+        __builtin_amdgcn_sched_barrier(0);  // keep each stage's loads right after its compute
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      comp(j);
+      if (s + P + j < nfull) load(j, s + P + j);
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j)
+      if (s + P + j < nfull) comp(j);
+  } else {
+    for (int64_t s = 0; s < nfull; ++s) {
+      load(0, s);
+      comp(0);
+    }
+  }
+  if (kb + nfull * kDotKStep < ke) {
+    // partial last step: guarded A, B is zero padded past K
+    const int64_t k0 = kb + nfull * kDotKStep, kk = k0 + 4 * g;
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+      a[0][t][0] = load4<VEC>(arow[t] + kk, ke - kk);
+      a[0][t][1] = load4<VEC>(arow[t] + kk + 16, ke - kk - 16);
+    }
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      b[0][c][0] = bp[nfull * bstep + c * 128];
+      b[0][c][1] = bp[nfull * bstep + c * 128 + 64];
+    }
+    comp(0);
   }
 
 #pragma unroll
@@ -155,6 +211,61 @@ __global__ __launch_bounds__(256) void k_skinny_dot(SkinnyParams p) {
       }
 }
 
+template <int RT, int CT, bool VEC>
+__global__ __launch_bounds__(256) void k_skinny_dot(SkinnyParams p) {
+  const int wpb = blockDim.x >> 6;
+  const int64_t task = (int64_t)blockIdx.x * wpb + (threadIdx.x >> 6);
+  const int64_t rows_per_wave = (int64_t)kDotRowsPerTile * RT;
+  const int64_t nrb = (p.M + rows_per_wave - 1) / rows_per_wave;
+  const int64_t rb = task / p.ksplit;
+  const int ks = (int)(task - rb * p.ksplit);
+  if (rb >= nrb) return;  // whole wave leaves; the kernel has no barriers
+  dot_tile<RT, CT, VEC>(p, rb, ks, blockIdx.y * CT);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent grouped SGEMM: hundreds of independent skinny problems in one
+// launch.  The host builds one descriptor per problem and a list of
+// (problem, row block) tiles sorted by decreasing cost; wave w of the
+// persistent grid takes tiles w, w + W, w + 2W, ... (W = waves launched), a
+// static longest-first schedule that needs no atomics or counters, so the
+// launch can be captured in a graph and replayed.  Each tile runs the same
+// pipelined DOT body as the single-problem kernel (16 rows x every column
+// of its problem, whole K, no split), dispatching on the problem's column
+// tile count and alignment.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_grouped_dot(const GroupProblem *probs,
+                                                     const GroupTile *tiles,
+                                                     int64_t ntiles) {
+  const int64_t W = (int64_t)gridDim.x * (blockDim.x >> 6);
+  const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  for (int64_t t = w; t < ntiles; t += W) {
+    const GroupTile tile = tiles[t];
+    const GroupProblem &gp = probs[tile.prob];
+    SkinnyParams p{};
+    p.A = gp.A; p.sAm = gp.lda; p.sAk = 1;
+    p.Bp = gp.Bp;
+    p.C = gp.C; p.sCm = gp.ldc; p.sCn = 1;
+    p.P = nullptr;
+    p.M = gp.M; p.N = gp.N; p.K = gp.K;
+    p.kchunk = gp.K > 0 ? gp.K : 1; p.ksplit = 1;
+    p.nct_pad = gp.nct_pad;
+    p.alpha = gp.alpha; p.beta = gp.beta; p.read_c = gp.read_c;
+    const int kind = gp.ct * 2 + gp.vec;
+    for (int ct0 = 0; ct0 < gp.nct_pad; ct0 += gp.ct) {
+      switch (kind) {
+        case 2: dot_tile<1, 1, false>(p, tile.rb, 0, ct0); break;
+        case 3: dot_tile<1, 1, true>(p, tile.rb, 0, ct0); break;
+        case 4: dot_tile<1, 2, false>(p, tile.rb, 0, ct0); break;
+        case 5: dot_tile<1, 2, true>(p, tile.rb, 0, ct0); break;
+        case 8: dot_tile<1, 4, false>(p, tile.rb, 0, ct0); break;
+        case 9: dot_tile<1, 4, true>(p, tile.rb, 0, ct0); break;
+        default: break;
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // AXPY form: A(m,k) = A[m + k*sAk].  A row tile is 64 rows: lane (g, r)
 // loads A(m0 + 4r .. m0 + 4r + 3, k) for k = k0 + 4u + g (u = 0..3 per
@@ -162,9 +273,13 @@ __global__ __launch_bounds__(256) void k_skinny_dot(SkinnyParams p) {
 // the interleaved rows m0 + 4r + j; its B operand B(k0 + 4u + g, 16c + r)
 // is shared by the 4 MFMAs.  Packed B fragment (step s, tile c) holds, for
 // lane (g, r), B(16s + 4u + g, 16c + r) for u = 0..3.
+// VEC requires M % 4 == 0, so a lane's 4 rows are all in range or all out;
+// out-of-range lanes load row 0 and multiply zeros.  The scalar variant
+// clamps and zeroes each element on its own.
 // ---------------------------------------------------------------------------
 template <int RT, int CT, bool VEC>
 __global__ __launch_bounds__(256) void k_skinny_axpy(SkinnyParams p) {
+  constexpr int P = (RT == 1 && CT <= 2) ? 3 : 2;
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
   const int64_t task = (int64_t)blockIdx.x * wpb + (threadIdx.x >> 6);
@@ -176,18 +291,28 @@ __global__ __launch_bounds__(256) void k_skinny_axpy(SkinnyParams p) {
   const int ct0 = blockIdx.y * CT;
   const int r = lane & 15, g = lane >> 4;
 
-  int64_t m4[RT], mvalid[RT];
+  // per lane: 4 rows m4[t] .. m4[t] + 3 of row tile t
+  int64_t m4[RT];
+  bool rin[RT][4];
+  const float *arow[RT][4];
 #pragma unroll
   for (int t = 0; t < RT; ++t) {
     m4[t] = rb * rows_per_wave + t * 64 + 4 * r;
-    mvalid[t] = p.M - m4[t];
-    if (mvalid[t] < 0) { mvalid[t] = 0; m4[t] = 0; }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      rin[t][e] = m4[t] + e < p.M;
+      arow[t][e] = p.A + (rin[t][e] ? m4[t] + e : (VEC ? 0 : p.M - 1));
+    }
   }
   const int64_t kb = (int64_t)ks * p.kchunk;
   int64_t ke = kb + p.kchunk;
   if (ke > p.K) ke = p.K;
+  const int64_t nfull = ke > kb ? (ke - kb) / kAxpyKStep : 0;
 
   const f4 *Bp = reinterpret_cast<const f4 *>(p.Bp);
+  const int64_t bstep = (int64_t)p.nct_pad * 64;
+  const f4 *bp = Bp + (kb / kAxpyKStep) * bstep + (int64_t)ct0 * 64 + lane;
+
   f4 acc[RT][4][CT];
 #pragma unroll
   for (int t = 0; t < RT; ++t)
@@ -196,45 +321,86 @@ __global__ __launch_bounds__(256) void k_skinny_axpy(SkinnyParams p) {
 #pragma unroll
       for (int c = 0; c < CT; ++c) acc[t][j][c] = f4{0.f, 0.f, 0.f, 0.f};
 
-  f4 a[RT][4], b[CT];
-  auto load_step = [&](int64_t k0, f4 (&aa)[RT][4], f4 (&bb)[CT]) {
+  // A values of row tile t at column k (zero where the row is outside M)
+  auto ldA = [&](int t, int64_t k) -> f4 {
+    f4 v;
+    if (VEC) {
+      v = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(arow[t][0] + k * p.sAk));
+      if (!rin[t][0]) v = f4{0.f, 0.f, 0.f, 0.f};
+    } else {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int64_t k = k0 + 4 * u + g;
-      const bool kin = k < ke;
-#pragma unroll
-      for (int t = 0; t < RT; ++t)
-        aa[t][u] = load4<VEC>(p.A + m4[t] + (kin ? k : 0) * p.sAk,
-                              kin ? mvalid[t] : 0);
+      for (int e = 0; e < 4; ++e) {
+        const float x = __builtin_nontemporal_load(arow[t][e] + k * p.sAk);
+        v[e] = rin[t][e] ? x : 0.f;
+      }
     }
-    const int64_t s = k0 / kAxpyKStep;
-#pragma unroll
-    for (int c = 0; c < CT; ++c)
-      bb[c] = Bp[(s * p.nct_pad + ct0 + c) * 64 + lane];
+    return v;
   };
 
-  if (kb < ke) load_step(kb, a, b);
-  for (int64_t k0 = kb; k0 < ke; k0 += kAxpyKStep) {
-    f4 an[RT][4], bn[CT];
-    const bool more = k0 + kAxpyKStep < ke;
-    if (more) load_step(k0 + kAxpyKStep, an, bn);
+  f4 a[P][RT][4], b[P][CT];
+  auto load = [&](int j, int64_t s) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t k = kb + s * kAxpyKStep + 4 * u + g;
+#pragma unroll
+      for (int t = 0; t < RT; ++t) a[j][t][u] = ldA(t, k);
+    }
+#pragma unroll
+    for (int c = 0; c < CT; ++c) b[j][c] = bp[s * bstep + c * 64];
+  };
+  auto comp = [&](int j) {
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
       for (int t = 0; t < RT; ++t)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int jj = 0; jj < 4; ++jj)
 #pragma unroll
           for (int c = 0; c < CT; ++c)
-            acc[t][j][c] = mfma4(a[t][u][j], b[c][u], acc[t][j][c]);
-    if (more) {
+            acc[t][jj][c] = mfma4(a[j][t][u][jj], b[j][c][u], acc[t][jj][c]);
+  };
+
+  if (nfull >= P) {
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      load(j, j);
+      __builtin_amdgcn_sched_barrier(0);  // issue the stages in order
+    }
+    int64_t s = 0;
+    for (; s + 2 * P <= nfull; s += P) {
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        comp(j);
+        load(j, s + P + j);
+        __builtin_amdgcn_sched_barrier(0);  // keep each stage's loads right after its compute
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      comp(j);
+      if (s + P + j < nfull) load(j, s + P + j);
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j)
+      if (s + P + j < nfull) comp(j);
+  } else {
+    for (int64_t s = 0; s < nfull; ++s) {
+      load(0, s);
+      comp(0);
+    }
+  }
+  if (kb + nfull * kAxpyKStep < ke) {
+    // partial last step: k beyond ke reads as zero
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t k = kb + nfull * kAxpyKStep + 4 * u + g;
 #pragma unroll
       for (int t = 0; t < RT; ++t)
-#pragma unroll
-        for (int u = 0; u < 4; ++u) a[t][u] = an[t][u];
-#pragma unroll
-      for (int c = 0; c < CT; ++c) b[c] = bn[c];
+        a[0][t][u] = k < ke ? ldA(t, k) : f4{0.f, 0.f, 0.f, 0.f};
     }
+#pragma unroll
+    for (int c = 0; c < CT; ++c) b[0][c] = bp[nfull * bstep + c * 64];
+    comp(0);
   }
 
 #pragma unroll
@@ -254,11 +420,10 @@ __global__ __launch_bounds__(256) void k_skinny_axpy(SkinnyParams p) {
 // ---------------------------------------------------------------------------
 // B packing.  One thread writes one 128-bit fragment element.
 // ---------------------------------------------------------------------------
-__global__ void k_pack_b(const float *__restrict__ B, int64_t sBk, int64_t sBn,
-                         int64_t K, int64_t N, int form, int64_t total,
-                         int nct_pad, f4 *__restrict__ Bp) {
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= total) return;
+__device__ __forceinline__ f4 pack_elem(const float *__restrict__ B,
+                                        int64_t sBk, int64_t sBn, int64_t K,
+                                        int64_t N, int form, int nct_pad,
+                                        int64_t idx) {
   const int lane = (int)(idx & 63);
   const int r = lane & 15, g = lane >> 4;
   f4 v = {0.f, 0.f, 0.f, 0.f};
@@ -269,7 +434,7 @@ __global__ void k_pack_b(const float *__restrict__ B, int64_t sBk, int64_t sBn,
     const int64_t sc = q >> 1;
     const int64_t c = sc % nct_pad, s = sc / nct_pad;
     n = c * 16 + r;
-    kbase = s * kDotKStep + 8 * g + 4 * half;
+    kbase = s * kDotKStep + 16 * half + 4 * g;
     kstride = 1;
   } else {
     const int64_t sc = idx >> 6;        // s*nct_pad + c
@@ -285,7 +450,26 @@ __global__ void k_pack_b(const float *__restrict__ B, int64_t sBk, int64_t sBn,
       if (k < K) v[j] = B[k * sBk + n * sBn];
     }
   }
-  Bp[idx] = v;
+  return v;
+}
+
+__global__ void k_pack_b(const float *__restrict__ B, int64_t sBk, int64_t sBn,
+                         int64_t K, int64_t N, int form, int64_t total,
+                         int nct_pad, f4 *__restrict__ Bp) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  Bp[idx] = pack_elem(B, sBk, sBn, K, N, form, nct_pad, idx);
+}
+
+// blockIdx.y = problem; blocks past a problem's packed size exit.
+__global__ void k_pack_group(const GroupPack *__restrict__ packs) {
+  const GroupPack pk = packs[blockIdx.y];
+  const int64_t nsteps = (pk.K + kDotKStep - 1) / kDotKStep;
+  const int64_t total = nsteps * pk.nct_pad * 128;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  reinterpret_cast<f4 *>(pk.Bp)[idx] =
+      pack_elem(pk.B, pk.sBk, pk.sBn, pk.K, pk.N, kFormDot, pk.nct_pad, idx);
 }
 
 // ---------------------------------------------------------------------------
@@ -409,7 +593,9 @@ RunFn find_skinny(int form, int rt, int ct, bool vec) {
   } else if (form == kFormAxpy) {
     switch (rt) {
       case 1: HML_CT_CASES(axpy_fn, 1)
-      case 2: HML_CT_CASES(axpy_fn, 2)
+      case 2:
+        if (ct == 8) return nullptr;  // 2 x 64 rows x 128 columns spills
+        HML_CT_CASES(axpy_fn, 2)
       default: return nullptr;
     }
   }
@@ -482,6 +668,29 @@ hipError_t launch_fill(float *dst, size_t n, float value, hipStream_t st) {
   if (nblk > 65536) nblk = 65536;
   hipLaunchKernelGGL(k_fill, dim3((unsigned)nblk), dim3(256), 0, st, dst, n,
                      value);
+  return hipGetLastError();
+}
+
+}  // namespace hml
+
+namespace hml {
+
+hipError_t launch_grouped(const GroupProblem *probs, const GroupTile *tiles,
+                          int64_t ntiles, int nblocks, hipStream_t st) {
+  if (ntiles <= 0) return hipSuccess;
+  if (nblocks < 1) nblocks = 1;
+  hipLaunchKernelGGL(k_grouped_dot, dim3((unsigned)nblocks), dim3(256), 0, st,
+                     probs, tiles, ntiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_group(const GroupPack *packs, int32_t count,
+                             int64_t max_elems, hipStream_t st) {
+  if (count <= 0 || max_elems <= 0) return hipSuccess;
+  const int64_t nblk = (max_elems + 255) / 256;
+  if (nblk > 0x7fffffff || count > 65535) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL(k_pack_group, dim3((unsigned)nblk, (unsigned)count),
+                     dim3(256), 0, st, packs);
   return hipGetLastError();
 }
 

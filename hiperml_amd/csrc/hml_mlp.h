@@ -1,0 +1,38 @@
+// hml_mlp.h -- learned kernel selector: MLP kernels and class space
+// (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/hml_ext.h"
+
+namespace hml {
+
+hipError_t mlp_linear_fwd(const float *X, const float *W, const float *bias,
+                          float *Y, int B, int I, int O, int relu,
+                          hipStream_t st);
+hipError_t mlp_softmax_xent(const float *Z, const float *T, const float *mask,
+                            float *probs, float *dZ, float *loss_row, int B,
+                            int O, hipStream_t st);
+hipError_t mlp_linear_bwd_data(const float *dZ, const float *W,
+                               const float *Xact, float *dX, int B, int I,
+                               int O, hipStream_t st);
+hipError_t mlp_linear_bwd_weight(const float *X, const float *dZ, float *dW,
+                                 float *db, int B, int I, int O,
+                                 hipStream_t st);
+hipError_t mlp_adam(float *w, const float *g, float *m, float *v, int n,
+                    float lr_t, float b1, float b2, float eps, hipStream_t st);
+hipError_t mlp_sum(const float *x, int n, float *out, hipStream_t st);
+hipError_t mlp_gather_rows(const float *src, const int32_t *idx, float *dst,
+                           int rows, int cols, hipStream_t st);
+
+// Selector class space.  Classes 0..83: skinny MFMA variants (rt index 0..2
+// -> 1/2/4 row tiles, column mode 0..1 -> full / half the column tiles that
+// cover N, ksplit 2^0..2^6, wpb 2 or 4); 84: dense FP32 tiles; 85: the GEMV
+// kernels (heuristic parameters).  The family of a skinny class follows the
+// problem's form (dot or axpy).
+constexpr int kSelFeatures = 6;
+constexpr int kSelClasses = 86;
+
+}  // namespace hml

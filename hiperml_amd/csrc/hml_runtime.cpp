@@ -123,6 +123,10 @@ hmlStatus hmlInit(void) {
   return HML_STATUS_SUCCESS;
 }
 
+hmlStatus hmlSetDevice(int device) {
+  return from_hip(hipSetDevice(device), "hipSetDevice");
+}
+
 int hmlDeviceCount(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;

@@ -3,6 +3,8 @@
 // dispatch, the autotuner and the config store.
 #pragma once
 
+#include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "../../include/hml_ext.h"
@@ -27,6 +29,7 @@ struct GemmKey {
   int beta0;     // beta == 0 and C is not read
   int transA, transB, order;  // caller's layout (features of the learned selector)
   bool gemv_ok;  // N == 1 and x / y are unit-stride, GEMV kernels apply
+  int64_t cM, cN;  // caller's M and N (before orientation)
 };
 
 struct GemvKey {
@@ -49,6 +52,21 @@ GemvKey gemv_key(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
 int ilog2_floor(int64_t x);
 
 void heuristic_sgemm(const GemmKey &k, hmlSgemmVariant *v);
+
+// Dense families (plain FP32 / bf16x3 / bf16x6 MFMA tiles).
+inline bool is_dense_family(int f) {
+  return f == HML_SGEMM_FAMILY_DENSE || f == HML_SGEMM_FAMILY_BF16X3 ||
+         f == HML_SGEMM_FAMILY_BF16X6;
+}
+inline int dense_precision(int f) {
+  return f == HML_SGEMM_FAMILY_BF16X3 ? 1 : f == HML_SGEMM_FAMILY_BF16X6 ? 2 : 0;
+}
+size_t dense_workspace(int prec, int64_t M, int64_t N, int64_t K);
+hmlStatus run_dense(int prec, const float *A, int64_t sAm, int64_t sAk,
+                    const float *B, int64_t sBk, int64_t sBn, float *C,
+                    int64_t sCm, int64_t sCn, int64_t M, int64_t N, int64_t K,
+                    float alpha, float beta, int read_c, void *ws,
+                    size_t ws_bytes, hipStream_t st);
 void heuristic_sgemv(const GemvKey &k, hmlSgemvVariant *v);
 void basic_sgemm(const GemmKey &k, hmlSgemmVariant *v);
 void basic_sgemv(const GemvKey &k, hmlSgemvVariant *v);
@@ -63,7 +81,12 @@ bool sgemv_variant_runnable(const GemvKey &k, const hmlSgemvVariant &v);
 bool config_select_sgemm(const GemmKey &k, hmlSgemmVariant *v, int *source);
 bool config_select_sgemv(const GemvKey &k, hmlSgemvVariant *v, int *source);
 
-// Selection entry used by the dispatch (config -> heuristic).
+// Learned MLP selection (when a trained net is installed).
+bool learned_select_sgemm(const GemmKey &k, hmlSgemmVariant *v);
+// Selector class -> variant for a key (false: the class cannot run it).
+bool selector_variant(const GemmKey &k, int cls, hmlSgemmVariant *v);
+
+// Selection entry used by the dispatch (learned -> config -> heuristic).
 void select_sgemm(const GemmKey &k, hmlSgemmVariant *v, int *source);
 void select_sgemv(const GemvKey &k, hmlSgemvVariant *v, int *source);
 

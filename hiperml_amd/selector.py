@@ -1,0 +1,244 @@
+"""Learned kernel selector: data generation, training and evaluation.
+
+The pipeline (all timings and all training run on the GPU through libhml):
+  1. random_shapes(): M, K log-uniform in [16, 65536], N log-uniform in
+     [1, 512], random transA/transB/beta==0, rejecting problems whose
+     autotune working set exceeds the memory cap;
+  2. autotune(): the exhaustive, validating autotuner (hmlSgemmAutotune)
+     times every candidate variant of each shape into a CSV database;
+  3. dataset(): per shape, the selector features, the mask of classes the
+     autotuner measured as valid, and soft targets
+     T_c ~ exp(-(t_c / t_best - 1) / tau) (tau = 0.05: a class 5% slower
+     than the best gets weight 1/e);
+  4. train(): the MLP (6 -> 3 x 64 ReLU -> classes) trained by the HIP
+     kernels (hmlMlpTrain);
+  5. evaluate(): on held-out shapes, the fraction of autotune-best
+     throughput reached by the MLP's pick, by the built-in heuristic and by
+     a nearest-bucket table built from the training shapes' winners.
+"""
+import csv
+import ctypes
+import math
+
+import numpy as np
+
+from . import (GemmShape, Mlp, SgemmVariant, check, config_clear, lib,
+               sgemm_select, selector_class_of, selector_features,
+               selector_mask)
+
+TAU = 0.05
+
+
+def autotune_bytes(M, N, K):
+    # A, B, C0, C, reference C and the largest workspace (packed B + partials)
+    return 4.0 * (M * K + K * N + 3 * M * N) + 4.0 * (K * max(N, 16) * 2 + 64 * M * N)
+
+
+def random_shapes(n, seed, mem_cap=8e9, lo=16, hi=65536, nlo=1, nhi=512):
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < n:
+        M = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        K = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        N = int(round(math.exp(rng.uniform(math.log(nlo), math.log(nhi)))))
+        if autotune_bytes(M, N, K) > mem_cap:
+            continue
+        ta, tb, b0 = (int(x) for x in rng.integers(0, 2, 3))
+        out.append((M, N, K, ta, tb, b0))
+    return out
+
+
+def autotune(shapes, db_path, trials=3, flush=True, seed=1, progress=None):
+    """Runs the exhaustive autotuner on every shape; returns the number of
+    shapes that produced a winner."""
+    ok = 0
+    for i, (M, N, K, ta, tb, b0) in enumerate(shapes):
+        s = GemmShape(M, N, K, 0, ta, tb, b0)
+        best = SgemmVariant()
+        bt, ht = ctypes.c_double(), ctypes.c_double()
+        st = lib.hmlSgemmAutotune(ctypes.byref(s), trials, 1 if flush else 0, seed + i,
+                                  ctypes.byref(best), ctypes.byref(bt), ctypes.byref(ht),
+                                  db_path.encode())
+        ok += st == 0
+        if progress and (i + 1) % progress == 0:
+            print("autotuned %d/%d shapes" % (i + 1, len(shapes)), flush=True)
+    return ok
+
+
+def load_db(path):
+    """{(M,N,K,transA,transB,beta0): {variant tuple: median ms}} (valid
+    SGEMM rows only)."""
+    db = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["kind"] != "0" or r["valid"] != "1":
+                continue
+            key = (int(r["M"]), int(r["N"]), int(r["K"]), int(r["transA"]),
+                   int(r["transB"]), int(r["beta0"]))
+            v = (int(r["family"]), int(r["p1"]), int(r["p2"]), int(r["p3"]), int(r["p4"]))
+            db.setdefault(key, {})[v] = float(r["median_ms"])
+    return db
+
+
+def shape_kw(key):
+    M, N, K, ta, tb, b0 = key
+    return dict(transA=bool(ta), transB=bool(tb), beta0=bool(b0))
+
+
+def class_times(key, times):
+    """Per selector class: measured median ms (inf where not measured)."""
+    M, N, K = key[:3]
+    C = lib.hmlSelectorClassCount()
+    t = np.full(C, np.inf)
+    for v, ms in times.items():
+        c = selector_class_of(M, N, K, v, **shape_kw(key))
+        if c >= 0:
+            t[c] = min(t[c], ms)
+    return t
+
+
+def dataset(db, tau=TAU):
+    keys = sorted(db)
+    X, T, Mk, TT = [], [], [], []
+    for key in keys:
+        M, N, K = key[:3]
+        t = class_times(key, db[key])
+        mask = (np.isfinite(t) & (selector_mask(M, N, K, **shape_kw(key)) > 0)).astype(np.float32)
+        if mask.sum() == 0:
+            continue
+        tb = t[mask > 0].min()
+        w = np.where(mask > 0, np.exp(-(np.where(mask > 0, t, tb) / tb - 1.0) / tau), 0.0)
+        X.append(selector_features(M, N, K, **shape_kw(key)))
+        T.append((w / w.sum()).astype(np.float32))
+        Mk.append(mask)
+        TT.append(t)
+    return (np.array(keys), np.array(X, np.float32), np.array(T, np.float32),
+            np.array(Mk, np.float32), np.array(TT))
+
+
+def train(X, T, mask, epochs=300, batch=256, lr=2e-3, seed=7, hidden=64, layers=3):
+    net = Mlp(X.shape[1], hidden, layers, T.shape[1], seed=seed)
+    losses = net.train(X, T, mask, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    return net, losses
+
+
+def best_time(times):
+    return min(times.values())
+
+
+def evaluate(net, db_eval, db_train=None):
+    """Fraction of autotune-best throughput (t_best / t_pick) per held-out
+    shape for the MLP, the heuristic and (given training data) the
+    nearest-bucket table of training winners."""
+    keys, X, T, mask, tcls = dataset(db_eval)
+    P = net.predict(X, mask) if net is not None else None
+    res = {"mlp": [], "heuristic": [], "table": []}
+    for i, key in enumerate(keys):
+        key = tuple(int(v) for v in key)
+        M, N, K, ta, tb, b0 = key
+        times = db_eval[key]
+        tbest = best_time(times)
+        if P is not None:
+            c = int(np.argmax(np.where(mask[i] > 0, P[i], -1.0)))
+            res["mlp"].append(tbest / tcls[i][c])
+        config_clear()
+        h, _ = sgemm_select(M, N, K, transA=bool(ta), transB=bool(tb), beta=0.0 if b0 else 1.0)
+        res["heuristic"].append(tbest / times.get(h.astuple(), math.inf))
+    if db_train is not None:
+        heur = {}
+        config_clear()
+        for key in keys:
+            key = tuple(int(v) for v in key)
+            M, N, K, ta, tb, b0 = key
+            heur[key] = sgemm_select(M, N, K, transA=bool(ta), transB=bool(tb),
+                                     beta=0.0 if b0 else 1.0)[0].astuple()
+        install_table(db_train)
+        for key in keys:
+            key = tuple(int(v) for v in key)
+            M, N, K, ta, tb, b0 = key
+            times = db_eval[key]
+            v, _ = sgemm_select(M, N, K, transA=bool(ta), transB=bool(tb), beta=0.0 if b0 else 1.0)
+            # a rule variant this shape's search never ran (a GEMV setting
+            # of another bucket) counts as the heuristic's time
+            t = times.get(v.astuple(), times.get(heur[key], math.inf))
+            res["table"].append(best_time(times) / t)
+        config_clear()
+    return {k: np.array(v) for k, v in res.items() if v}
+
+
+def install_table(db_train):
+    """v2 config rules (one per shape bucket) from the training winners."""
+    config_clear()
+    for key, times in sorted(db_train.items()):
+        M, N, K, ta, tb, b0 = key
+        v = min(times, key=times.get)
+        s = GemmShape(M, N, K, 0, ta, tb, b0)
+        lib.hmlSgemmConfigAddRule(ctypes.byref(s), ctypes.byref(SgemmVariant(*v)))
+
+
+def run(out_dir, n_train, n_eval, seed=2024, epochs=300, trials=3, mem_cap=8e9,
+        train_db=None, eval_db=None):
+    """End-to-end: autotune random training and held-out shapes (unless
+    databases are given), train the MLP on the GPU, evaluate, save the model
+    (hml_selector_mi355x.bin) and a JSON report into out_dir."""
+    import json
+    import os
+    import time
+    os.makedirs(out_dir, exist_ok=True)
+    t0 = time.time()
+    if train_db is None:
+        train_db = os.path.join(out_dir, "autotune_train.csv")
+        shapes = random_shapes(n_train, seed, mem_cap)
+        autotune(shapes, train_db, trials=trials, seed=seed, progress=25)
+        print("train autotune done in %.0f s" % (time.time() - t0), flush=True)
+    if eval_db is None:
+        eval_db = os.path.join(out_dir, "autotune_eval.csv")
+        shapes = random_shapes(n_eval, seed + 1, mem_cap)
+        autotune(shapes, eval_db, trials=trials, seed=seed + 100000, progress=50)
+        print("eval autotune done in %.0f s" % (time.time() - t0), flush=True)
+    dbt, dbe = load_db(train_db), load_db(eval_db)
+    keys, X, T, mask, _ = dataset(dbt)
+    timings = sum(len(v) for v in dbt.values())
+    print("training set: %d shapes, %d timings" % (len(keys), timings), flush=True)
+    net, losses = train(X, T, mask, epochs=epochs, seed=seed)
+    fr = evaluate(net, dbe, dbt)
+    rep = {"train_shapes": len(keys), "train_timings": timings,
+           "eval_shapes": len(dbe), "epochs": epochs,
+           "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+           "fraction_of_autotune_best": summarize(fr),
+           "seconds": round(time.time() - t0, 1)}
+    net.save(os.path.join(out_dir, "hml_selector_mi355x.bin"))
+    with open(os.path.join(out_dir, "selector_report.json"), "w") as f:
+        json.dump(rep, f, indent=1)
+    print(json.dumps(rep), flush=True)
+    return rep
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="train and evaluate the learned kernel selector")
+    ap.add_argument("--out", default="selector_out")
+    ap.add_argument("--train", type=int, default=500, help="random training shapes")
+    ap.add_argument("--eval", type=int, default=2000, help="random held-out shapes")
+    ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--trials", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--mem-cap", type=float, default=8e9)
+    ap.add_argument("--train-db")
+    ap.add_argument("--eval-db")
+    a = ap.parse_args(argv)
+    check(lib.hmlInit())
+    run(a.out, a.train, a.eval, a.seed, a.epochs, a.trials, a.mem_cap, a.train_db, a.eval_db)
+
+
+def summarize(fracs):
+    out = {}
+    for k, a in fracs.items():
+        a = np.asarray(a, float)
+        out[k] = {"mean": float(a.mean()), "geomean": float(np.exp(np.log(np.maximum(a, 1e-9)).mean())),
+                  "min": float(a.min()), "within_5pct": float((a >= 0.95).mean()), "n": int(a.size)}
+    return out
+
+
+if __name__ == "__main__":
+    main()

@@ -52,8 +52,9 @@ typedef enum {
   HML_SGEMM_FAMILY_DOT    = 1, /* MFMA f32, streamed operand contiguous in K */
   HML_SGEMM_FAMILY_AXPY   = 2, /* MFMA f32, streamed operand contiguous in M */
   HML_SGEMM_FAMILY_GEMV   = 3, /* N == 1 routed to the GEMV kernels        */
-  HML_SGEMM_FAMILY_BF16X3 = 4, /* split-precision bf16 MFMA (dense)        */
-  HML_SGEMM_FAMILY_DENSE  = 5  /* plain FP32 MFMA dense tile kernel        */
+  HML_SGEMM_FAMILY_BF16X3 = 4, /* split-precision bf16 MFMA, 2-term split  */
+  HML_SGEMM_FAMILY_DENSE  = 5, /* plain FP32 MFMA dense tile kernel        */
+  HML_SGEMM_FAMILY_BF16X6 = 6  /* split-precision bf16 MFMA, 3-term split  */
 } hmlSgemmFamily;
 
 typedef struct {
@@ -113,6 +114,44 @@ size_t hmlSgemmWorkspaceSize(hmlOrder order, hmlOp transA, hmlOp transB,
 hmlStatus hmlSgemmSelect(hmlOrder order, hmlOp transA, hmlOp transB,
                          int64_t M, int64_t N, int64_t K, float beta,
                          hmlSgemmVariant *variant, int32_t *source);
+
+/* Dense SGEMM on the matrix cores, no reference counterpart (the reference
+ * has only its skinny kernels).  precision: 0 = plain FP32 MFMA
+ * (v_mfma_f32_16x16x4f32), 1 = bf16x3 (FP32 inputs split into hi/lo bf16,
+ * products hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16, FP32
+ * accumulation), 2 = bf16x6 (hi/mid/lo split, the six products down to
+ * 2^-16 relative weight).  Other arguments as hmlSgemmEx; the workspace holds
+ * the split planes (hmlSgemmDenseWorkspaceSize). */
+hmlStatus hmlSgemmDense(int32_t precision, hmlOrder order, hmlOp transA,
+                        hmlOp transB, int64_t M, int64_t N, int64_t K,
+                        float alpha, const float *A, int64_t lda,
+                        const float *B, int64_t ldb, float beta, float *C,
+                        int64_t ldc, uint32_t flags, void *workspace,
+                        size_t ws_bytes, void *stream);
+size_t hmlSgemmDenseWorkspaceSize(int32_t precision, int64_t M, int64_t N,
+                                  int64_t K);
+
+/* Grouped skinny SGEMM, no reference counterpart: C_i = alpha * A_i * B_i +
+ * beta * C_i for `count` independent ROW-MAJOR problems (M_i >= N_i), given
+ * as host arrays of device pointers and extents.  The plan owns packed B
+ * storage, the device descriptors and a longest-first tile schedule for one
+ * persistent kernel (column tiles per problem from the selector).  Run
+ * packs every B_i and computes every C_i in two launches; with use_graph the
+ * pair is captured into a hipGraph on the first run and replayed after.
+ * The A/B/C pointers are captured at creation: rerun after changing their
+ * contents, create a new plan for new pointers. */
+typedef struct hmlGroupedPlan_t *hmlGroupedPlan;
+hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan *plan, int32_t count,
+                                const int64_t *M, const int64_t *N,
+                                const int64_t *K, const float *const *A,
+                                const int64_t *lda, const float *const *B,
+                                const int64_t *ldb, float *const *C,
+                                const int64_t *ldc, float alpha, float beta);
+hmlStatus hmlSgemmGroupedRun(hmlGroupedPlan plan, int32_t use_graph,
+                             void *stream);
+hmlStatus hmlSgemmGroupedInfo(hmlGroupedPlan plan, int64_t *ntiles,
+                              int32_t *nblocks, size_t *device_bytes);
+hmlStatus hmlSgemmGroupedDestroy(hmlGroupedPlan plan);
 
 /* y = alpha * op(A) * x + beta * y; A is M x N in `order`; incx/incy must
  * be 1. */
@@ -193,6 +232,54 @@ hmlStatus hmlSgemmConfigAddRule(const hmlGemmShape *shape,
 hmlStatus hmlSgemvConfigAddRule(hmlOrder order, hmlOp trans, int64_t M,
                                 int64_t N, const hmlSgemvVariant *variant);
 
+/* ---- learned kernel selector ---------------------------------------------
+ * No reference counterpart (the reference selects from a static table,
+ * src/hml_sgemm_kernel.cu:189-237).  A small FP32 MLP (n_in -> n_hidden
+ * ReLU layers of `hidden` -> n_out logits) whose training (masked
+ * soft-target cross-entropy, Adam) and inference run in HIP kernels.  All
+ * array arguments are HOST arrays, row-major [rows x columns]; calls are
+ * synchronous.  Weights are laid out per layer as W[in][out] then bias[out]. */
+typedef struct hmlMlp_t *hmlMlp;
+hmlStatus hmlMlpCreate(hmlMlp *net, int32_t n_in, int32_t hidden,
+                       int32_t n_hidden, int32_t n_out, uint64_t seed);
+hmlStatus hmlMlpDestroy(hmlMlp net);
+int64_t   hmlMlpParamCount(hmlMlp net);
+hmlStatus hmlMlpGetParams(hmlMlp net, float *params, int64_t count);
+hmlStatus hmlMlpSetParams(hmlMlp net, const float *params, int64_t count);
+/* Mean loss and the gradient of the summed-over-batch / batch loss for one
+ * batch (X: batch x n_in, T and mask: batch x n_out; mask 1 = class allowed,
+ * T sums to 1 over allowed classes). */
+hmlStatus hmlMlpLossGrad(hmlMlp net, const float *X, const float *T,
+                         const float *mask, int32_t batch, float *loss,
+                         float *grads);
+/* Shuffled minibatch Adam over `count` rows for `epochs`; epoch_loss (may
+ * be NULL) receives the mean loss of each epoch. */
+hmlStatus hmlMlpTrain(hmlMlp net, const float *X, const float *T,
+                      const float *mask, int64_t count, int32_t epochs,
+                      int32_t batch, float lr, uint64_t seed,
+                      float *epoch_loss);
+/* Masked softmax probabilities (count x n_out). */
+hmlStatus hmlMlpPredict(hmlMlp net, const float *X, const float *mask,
+                        int64_t count, float *probs);
+hmlStatus hmlMlpSave(hmlMlp net, const char *path);
+hmlStatus hmlMlpLoad(hmlMlp *net, const char *path);
+
+/* The selector's features (log2 M/16, log2 N/16, log2 K/16, transA,
+ * transB, beta==0 of the row-major equivalent problem) and its class space
+ * (skinny MFMA variants by row tiles, column tiles, K split and waves per
+ * block; dense FP32 tiles; GEMV kernels).  Mask: 1 for classes that can run
+ * the shape.  Install: the net becomes the first selector of hmlSgemm /
+ * hmlSgemmEx (decisions memoised per problem); NULL uninstalls. */
+int32_t   hmlSelectorFeatureCount(void);
+int32_t   hmlSelectorClassCount(void);
+hmlStatus hmlSelectorFeatures(const hmlGemmShape *shape, float *features);
+hmlStatus hmlSelectorMask(const hmlGemmShape *shape, float *mask);
+hmlStatus hmlSelectorVariant(const hmlGemmShape *shape, int32_t cls,
+                             hmlSgemmVariant *variant);
+int32_t   hmlSelectorClassOf(const hmlGemmShape *shape,
+                             const hmlSgemmVariant *variant);
+hmlStatus hmlSelectorInstall(hmlMlp net);
+
 /* dst[i] = lo + (hi - lo) * u_i with u_i from a counter-based splitmix64
  * stream of `seed` (device-side synthetic inputs). */
 hmlStatus hmlFillUniform(float *dst, size_t n, uint64_t seed, float lo,
@@ -215,6 +302,9 @@ hmlStatus hmlSgemmCheckF64(hmlOrder order, hmlOp transA, hmlOp transB,
 /* ---- device / memory helpers (so hosts need no other GPU runtime) -------- */
 hmlStatus hmlInit(void);
 int       hmlDeviceCount(void);
+/* Makes `device` current for the calling thread (the reference apps' -u,
+ * apps/gemm/hml_sgemm_main.cu:430). */
+hmlStatus hmlSetDevice(int device);
 /* name (>= 64 bytes), compute units, L2 bytes, HBM bytes, clock kHz */
 hmlStatus hmlDeviceInfo(int device, char *name, int32_t *cus,
                         int64_t *l2_bytes, int64_t *hbm_bytes,
