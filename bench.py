@@ -56,7 +56,17 @@ def main():
                     help="also run the grouped 512-problem suite (extra JSON line)")
     ap.add_argument("--only-extra", action="store_true",
                     help="skip the main sweep, run only the requested extra suites")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the sweep, run 3 device copies and 3 fills of 256 MiB "
+                         "(the byte-counter calibration kernels of a --pmc run)")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="no GPU work: turn two rocprofv3 --pmc counter_collection.csv "
+                         "files (FETCH_SIZE, WRITE_SIZE) of a --calibrate-copy run into "
+                         "profiles/traffic_gemv_dot_16384.json")
     args = ap.parse_args()
+    if args.traffic:
+        traffic_from_pmc(*args.traffic)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU "
               "path (--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -210,6 +220,16 @@ def main():
     }
     print(json.dumps(out))
     sys.stdout.flush()
+    if args.calibrate_copy:
+        n = 256 << 20
+        src = bufs["gemmA_0"]  # 8192 x 8192 floats = 256 MiB
+        dst = hml.DeviceBuffer(n)
+        for _ in range(3):
+            hml.check(hml.lib.hmlMemcpyDtoD(dst.ptr, src.ptr, n))
+        for i in range(3):
+            dst.fill_uniform(n // 4, 99 + i, stream=S)
+        stream.synchronize()
+        print("calibration: 3 copies + 3 fills of %d bytes" % n, file=sys.stderr)
     if args.dense:
         bench_dense(hml)
     if args.grouped:
@@ -322,6 +342,67 @@ def bench_grouped(hml, count=512, reps=5, seed=0):
                       "grouped_graph_GB_s": round(nbytes / t_graph / 1e6, 1),
                       "tiles": info["tiles"], "blocks": info["blocks"],
                       "max_err_over_bound_first64": worst}))
+
+
+CALIB_BYTES = 256 << 20
+
+
+def _pmc_rows(path, counter):
+    """(kernel name, grid size, counter value in bytes) per dispatch of a
+    rocprofv3 counter_collection.csv (FETCH_SIZE / WRITE_SIZE are KiB)."""
+    import csv
+    out = []
+    with open(path) as f:
+        rd = csv.DictReader(f)
+        cols = {c.lower(): c for c in rd.fieldnames}
+        kn, cn, cv = cols["kernel_name"], cols["counter_name"], cols["counter_value"]
+        gs = cols.get("grid_size") or cols.get("grid_size_x")
+        for r in rd:
+            if r[cn].strip() != counter:
+                continue
+            out.append((r[kn], int(float(r[gs])) if gs else 0, float(r[cv]) * 1024.0))
+    return out
+
+
+def traffic_from_pmc(fetch_csv, write_csv, dst=None):
+    """HBM bytes per launch of the SGEMV 16384^2 A x kernel, each counter
+    scaled by what it reads for a plain copy of CALIB_BYTES (copy:
+    CALIB_BYTES read and written) and checked against the fill kernel
+    (CALIB_BYTES written, nothing read)."""
+    res = {}
+    for name, path, counter in (("fetch", fetch_csv, "FETCH_SIZE"),
+                                ("write", write_csv, "WRITE_SIZE")):
+        rows = _pmc_rows(path, counter)
+        groups = {}
+        for kn, g, v in rows:
+            groups.setdefault((kn, g), []).append(v)
+        gemv = [(np.mean(v), k) for k, v in groups.items() if "gemv_dot" in k[0]]
+        copy = [(np.mean(v), k) for k, v in groups.items() if "copy" in k[0].lower()]
+        fill = [(np.mean(v), k) for k, v in groups.items() if "fill_uniform" in k[0]]
+        res[name] = {"gemv_bytes": max(gemv)[0] if gemv else None,
+                     "gemv_kernel": max(gemv)[1][0] if gemv else None,
+                     "copy_bytes": max(copy)[0] if copy else None,
+                     "copy_kernel": max(copy)[1][0] if copy else None,
+                     "fill_bytes": max(fill)[0] if fill else None}
+    f, w = res["fetch"], res["write"]
+    out = {"kernel": f["gemv_kernel"], "shape": "SGEMV A x, M = N = 16384",
+           "algorithmic_bytes_per_launch": gemv_bytes(16384, 16384),
+           "calibration_bytes": CALIB_BYTES, "counters": res}
+    if f["gemv_bytes"] and f["copy_bytes"] and w["gemv_bytes"] is not None and w["copy_bytes"]:
+        kf = CALIB_BYTES / f["copy_bytes"]
+        kw = CALIB_BYTES / w["copy_bytes"]
+        out["fetch_factor"] = kf
+        out["write_factor"] = kw
+        out["traffic_bytes_per_launch"] = f["gemv_bytes"] * kf + w["gemv_bytes"] * kw
+        out["traffic_over_algorithmic"] = out["traffic_bytes_per_launch"] / gemv_bytes(16384, 16384)
+    if dst is None:
+        here = os.path.dirname(os.path.abspath(__file__))
+        dst = os.path.join(here, "profiles", "traffic_gemv_dot_16384.json")
+    os.makedirs(os.path.dirname(dst), exist_ok=True)
+    with open(dst, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out))
+    return out
 
 
 def cpu_baseline(hml, calls, budget_s):

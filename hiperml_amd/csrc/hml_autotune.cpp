@@ -46,7 +46,7 @@ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 struct DevBuf {
   void *p = nullptr;
-  ~DevBuf() { if (p) hipFree(p); }
+  ~DevBuf() { if (p) (void)hipFree(p); }
   hmlStatus alloc(size_t bytes) {
     hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
     if (e != hipSuccess) {
@@ -62,8 +62,8 @@ struct DevBuf {
 struct Events {
   hipEvent_t a = nullptr, b = nullptr;
   ~Events() {
-    if (a) hipEventDestroy(a);
-    if (b) hipEventDestroy(b);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
   }
   hmlStatus init() {
     hipError_t e = hipEventCreate(&a);
@@ -332,12 +332,12 @@ hmlStatus autotune_sgemm_shape(const hmlGemmShape &s, int trials, int flush,
       std::vector<double> ts;
       for (int i = 0; i < trials; ++i) {
         if (flush && hmlFlushCaches(nullptr) != HML_STATUS_SUCCESS) return HML_STATUS_HIP_ERROR;
-        hipEventRecord(ev.a, nullptr);
+        if ((e = hipEventRecord(ev.a, nullptr)) != hipSuccess) return from_hip(e, "timing");
         if ((st = run(v, C.f()))) break;
-        hipEventRecord(ev.b, nullptr);
+        if ((e = hipEventRecord(ev.b, nullptr)) != hipSuccess) return from_hip(e, "timing");
         if ((e = hipEventSynchronize(ev.b)) != hipSuccess) return from_hip(e, "timing");
         float ms = 0.f;
-        hipEventElapsedTime(&ms, ev.a, ev.b);
+        if ((e = hipEventElapsedTime(&ms, ev.a, ev.b)) != hipSuccess) return from_hip(e, "timing");
         ts.push_back(ms);
       }
       if (st == HML_STATUS_SUCCESS) t = median(ts);
@@ -447,12 +447,12 @@ hmlStatus autotune_sgemv_shape(hmlOrder order, hmlOp trans, int64_t M,
       std::vector<double> ts;
       for (int i = 0; i < trials; ++i) {
         if (flush && hmlFlushCaches(nullptr) != HML_STATUS_SUCCESS) return HML_STATUS_HIP_ERROR;
-        hipEventRecord(ev.a, nullptr);
+        if ((e = hipEventRecord(ev.a, nullptr)) != hipSuccess) return from_hip(e, "timing");
         if ((st = run(v, y.f()))) break;
-        hipEventRecord(ev.b, nullptr);
+        if ((e = hipEventRecord(ev.b, nullptr)) != hipSuccess) return from_hip(e, "timing");
         if ((e = hipEventSynchronize(ev.b)) != hipSuccess) return from_hip(e, "timing");
         float ms = 0.f;
-        hipEventElapsedTime(&ms, ev.a, ev.b);
+        if ((e = hipEventElapsedTime(&ms, ev.a, ev.b)) != hipSuccess) return from_hip(e, "timing");
         ts.push_back(ms);
       }
       if (st == HML_STATUS_SUCCESS) t = median(ts);

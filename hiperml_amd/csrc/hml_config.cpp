@@ -103,7 +103,7 @@ const char kSgemmHeaderV2[] =
     "#  log2X  : floor(log2(X)) of the oriented problem\n"
     "#  family : 1 = MFMA dot, 2 = MFMA axpy, 3 = GEMV kernels (N == 1)\n"
     "#  rt, ct : row tiles / 16-column tiles per wave\n"
-    "#  ksplit : K split across workgroups, wpb : waves per workgroup\n";
+    "#  ksplit : K chunks per tile, wpb : waves per workgroup (dot: they split K)\n";
 
 const char kSgemvHeaderV2[] =
     "#SGEMV smart kernel config file v2.0 (MI355X gfx950)\n"
@@ -345,11 +345,17 @@ bool v1_sgemm_variant(const GemmKey &k, hmlSgemmVariant *v) {
   }
   heuristic_sgemm(k, v);
   if (v->family == HML_SGEMM_FAMILY_GEMV) return true;
+  const int ctcap = k.form == kFormDot ? 4 : 8;
   int ct = 1;
-  while (ct < 8 && ct < cs) ct <<= 1;
+  while (ct < ctcap && ct < cs) ct <<= 1;
   v->ct = ct;
-  if (k.form == kFormDot) v->rt = r.rowStops >= 8 ? 4 : r.rowStops >= 4 ? 2 : 1;
-  else v->rt = r.rowStops >= 8 ? 2 : 1;
+  if (k.form == kFormDot) {
+    v->rt = r.rowStops >= 8 ? 4 : r.rowStops >= 4 ? 2 : 1;
+    while (v->rt * v->ct > 8) v->rt >>= 1;
+    if (!skinny_dot_runnable(v->rt, v->ct, v->wpb)) v->wpb = 4;
+  } else {
+    v->rt = r.rowStops >= 8 ? 2 : 1;
+  }
   return true;
 }
 

@@ -115,29 +115,57 @@ GemmCanon dense_canon(hmlOrder order, hmlOp ta, hmlOp tb, int64_t M,
 
 namespace {
 
-int64_t kstep_of(int form) { return form == kFormDot ? kDotKStep : kAxpyKStep; }
-
 // Resolved launch geometry of a skinny variant.
+//  AXPY form: K split into `ksplit` chunks over waves, partials reduced by a
+//    second kernel; B packed into Bp (nct_pad column tiles).
+//  DOT form (k_skinny_dot): K split into chunks of kchunk (a multiple of the
+//    64-k stage), kw consecutive chunks per workgroup summed in LDS, and
+//    kgroups workgroups along K whose partials are reduced when > 1; B is
+//    read in place.  variant.wpb bounds kw.
 struct SkinnyPlan {
   int nct_pad;
   int64_t kchunk;
-  int ksplit;
+  int ksplit;   // partial sums to reduce (1: none)
+  int kw;       // DOT: waves per workgroup
   size_t bp_floats, p_floats;
 };
 
 SkinnyPlan plan_skinny(const GemmKey &k, const hmlSgemmVariant &v) {
   SkinnyPlan pl;
   const int64_t nct = ceil_div(std::max<int64_t>(k.N, 1), 16);
-  pl.nct_pad = (int)(ceil_div(nct, v.ct) * v.ct);
-  const int64_t step = kstep_of(k.form);
+  const int64_t Kc = std::max<int64_t>(k.K, 1);
   int64_t ks = std::max<int64_t>(1, v.ksplit);
-  int64_t kchunk = ceil_div(std::max<int64_t>(k.K, 1), ks);
-  kchunk = ceil_div(kchunk, step) * step;
-  pl.kchunk = kchunk;
-  pl.ksplit = (int)ceil_div(std::max<int64_t>(k.K, 1), kchunk);
-  pl.bp_floats = packed_b_floats(k.form, k.K, pl.nct_pad);
+  if (k.form == kFormDot) {
+    pl.nct_pad = (int)nct;
+    int64_t kchunk = ceil_div(ceil_div(Kc, ks), kDotStageK) * kDotStageK;
+    const int64_t nchunks = ceil_div(Kc, kchunk);
+    pl.kchunk = kchunk;
+    pl.kw = (int)std::max<int64_t>(1, std::min<int64_t>(std::max(1, v.wpb), nchunks));
+    pl.ksplit = (int)ceil_div(nchunks, pl.kw);
+    pl.bp_floats = 0;
+  } else {
+    pl.nct_pad = (int)(ceil_div(nct, v.ct) * v.ct);
+    int64_t kchunk = ceil_div(Kc, ks);
+    kchunk = ceil_div(kchunk, kAxpyKStep) * kAxpyKStep;
+    pl.kchunk = kchunk;
+    pl.ksplit = (int)ceil_div(Kc, kchunk);
+    pl.kw = 1;
+    pl.bp_floats = packed_b_floats(k.K, pl.nct_pad);
+  }
   pl.p_floats = pl.ksplit > 1 ? (size_t)pl.ksplit * k.M * k.N : 0;
   return pl;
+}
+
+// Every in-range byte offset a DOT-form wave forms from its buffer
+// descriptors stays below 2^31 (k_skinny_dot's addressing contract).
+bool dot_offsets_fit(const GemmCanon &g, const hmlSgemmVariant &v,
+                     const SkinnyPlan &pl) {
+  const double lim = 2147483647.0;
+  const double rows = (double)std::min<int64_t>(16 * v.rt, g.M);
+  const double cols = (double)std::min<int64_t>(16 * v.ct, g.N);
+  const double a = ((rows - 1) * (double)g.sAm + pl.kchunk + 4) * 4.0;
+  const double b = ((pl.kchunk + 16.0) * (double)g.sBk + (cols - 1) * (double)g.sBn) * 4.0;
+  return a < lim && b < lim;
 }
 
 GemvParams gemv_params_from_canon(const GemmCanon &g, int family, float alpha,
@@ -172,22 +200,28 @@ void heuristic_sgemm(const GemmKey &k, hmlSgemmVariant *v) {
     return;
   }
   const int64_t nct = ceil_div(k.N, 16);
-  v->ct = nct <= 1 ? 1 : nct <= 2 ? 2 : nct <= 4 ? 4 : 8;
-  v->wpb = 4;
-  int64_t rows_per_wave;
+  int64_t tiles;
   if (k.form == kFormDot) {
+    // k_skinny_dot: one tile is 16 rt rows x 16 ct columns; the 4 waves of
+    // a workgroup split K and sum in LDS.
     v->family = HML_SGEMM_FAMILY_DOT;
-    v->rt = (v->ct <= 2 && k.M >= 32768) ? 2 : 1;
-    rows_per_wave = 16 * v->rt;
+    v->ct = nct <= 1 ? 1 : nct <= 2 ? 2 : 4;
+    v->rt = k.M >= 2048 ? 2 : 1;
+    v->wpb = 4;
+    tiles = ceil_div(k.M, 16 * v->rt) * ceil_div(nct, v->ct);
   } else {
     v->family = HML_SGEMM_FAMILY_AXPY;
+    v->ct = nct <= 1 ? 1 : nct <= 2 ? 2 : nct <= 4 ? 4 : 8;
     v->rt = 1;
-    rows_per_wave = 64;
+    v->wpb = 4;
+    tiles = ceil_div(k.M, 64) * ceil_div(nct, v->ct);
   }
-  const int64_t rowblocks = ceil_div(k.M, rows_per_wave) * (ceil_div(nct, v->ct));
-  const int64_t target = 4096;
-  int64_t ks = rowblocks >= target ? 1 : ceil_div(target, rowblocks);
-  ks = std::min<int64_t>(ks, std::max<int64_t>(1, k.K / 512));
+  // About one resident round of waves (3 per SIMD of 256 CUs for the DOT
+  // kernel: 768 workgroups of 4 waves; the whole split then runs at once),
+  // each K chunk at least 256 long.
+  const int64_t target = k.form == kFormDot ? 3072 : 4096;
+  int64_t ks = tiles >= target ? 1 : (target + tiles / 2) / tiles;
+  ks = std::min<int64_t>(ks, std::max<int64_t>(1, k.K / 256));
   ks = std::min<int64_t>(ks, 64);
   v->ksplit = (int)std::max<int64_t>(1, ks);
 }
@@ -203,6 +237,10 @@ void basic_sgemm(const GemmKey &k, hmlSgemmVariant *v) {
 }
 
 bool sgemm_variant_runnable(const GemmKey &k, const hmlSgemmVariant &v) {
+  if (v.family == HML_SGEMM_FAMILY_DOT)
+    return k.form == kFormDot && v.ksplit >= 1 && v.ksplit <= 4096 &&
+           (v.wpb == 1 || v.wpb == 2 || v.wpb == 4 || v.wpb == 8) &&
+           skinny_dot_runnable(v.rt, v.ct, v.wpb);
   if (v.wpb != 1 && v.wpb != 2 && v.wpb != 4) return false;
   if (v.family == HML_SGEMM_FAMILY_GEMV) {
     if (!k.gemv_ok) return false;
@@ -212,7 +250,6 @@ bool sgemm_variant_runnable(const GemmKey &k, const hmlSgemmVariant &v) {
   }
   if (is_dense_family(v.family)) return true;
   if (v.ksplit < 1 || v.ksplit > 4096) return false;
-  if (v.family == HML_SGEMM_FAMILY_DOT) return k.form == kFormDot && skinny_has_variant(kFormDot, v.rt, v.ct);
   if (v.family == HML_SGEMM_FAMILY_AXPY) return k.form == kFormAxpy && skinny_has_variant(kFormAxpy, v.rt, v.ct);
   return false;
 }
@@ -265,11 +302,43 @@ hmlStatus run_sgemm(const GemmCanon &g, const GemmKey &k,
                      read_c, ws, ws ? need : 0, st);
 
   const SkinnyPlan pl = plan_skinny(k, v);
+  if (k.form == kFormDot) {
+    if (!dot_offsets_fit(g, v, pl)) {
+      // strides too large for 31-bit buffer offsets: dense tiles instead
+      // (they address with 64-bit pointers through the split pass)
+      const size_t dn = dense_workspace(0, k.M, k.N, k.K);
+      void *dws = nullptr;
+      hmlStatus s = library_workspace(dn, &dws);
+      if (s != HML_STATUS_SUCCESS) return s;
+      return run_dense(0, g.A, g.sAm, g.sAk, g.B, g.sBk, g.sBn, g.C, g.sCm, g.sCn,
+                       g.M, g.N, g.K, alpha, beta, read_c, dws, dn, st);
+    }
+    DotParams p{};
+    p.A = g.A; p.sAm = g.sAm;
+    p.B = g.B; p.sBk = g.sBk; p.sBn = g.sBn;
+    p.C = g.C; p.sCm = g.sCm; p.sCn = g.sCn;
+    p.P = (float *)ws;
+    p.M = g.M; p.N = g.N; p.K = g.K;
+    p.kchunk = pl.kchunk;
+    p.kgroups = pl.ksplit;
+    p.alpha = alpha; p.beta = beta; p.read_c = read_c;
+    const bool avec = ((uintptr_t)g.A % 16 == 0) && (g.sAm % 4 == 0 || g.M == 1);
+    const bool bvec = g.sBk == 1 && ((uintptr_t)g.B % 16 == 0) &&
+                      (g.sBn % 4 == 0 || g.N == 1);
+    hipError_t e = launch_skinny_dot(v.rt, v.ct, avec, bvec, pl.kw, p, st);
+    if (e != hipSuccess) return from_hip(e, "skinny kernel launch");
+    if (pl.ksplit > 1) {
+      e = launch_reduce(p.P, pl.ksplit, g.M * g.N, g.N, g.C, g.sCm, g.sCn, alpha,
+                        beta, read_c, st);
+      if (e != hipSuccess) return from_hip(e, "split-K reduce");
+    }
+    return HML_STATUS_SUCCESS;
+  }
   float *Bp = (float *)ws;
   float *P = (float *)((char *)ws + align256(pl.bp_floats * sizeof(float)));
-  const int64_t nsteps = ceil_div(k.K, kstep_of(k.form));
-  hipError_t e = launch_pack_b(g.B, g.sBk, g.sBn, g.K, g.N, k.form, nsteps,
-                               pl.nct_pad, Bp, st);
+  const int64_t nsteps = ceil_div(k.K, kAxpyKStep);
+  hipError_t e = launch_pack_b(g.B, g.sBk, g.sBn, g.K, g.N, nsteps, pl.nct_pad,
+                               Bp, st);
   if (e != hipSuccess) return from_hip(e, "pack B");
   SkinnyParams p{};
   p.A = g.A; p.sAm = g.sAm; p.sAk = g.sAk;
@@ -280,12 +349,8 @@ hmlStatus run_sgemm(const GemmCanon &g, const GemmKey &k,
   p.kchunk = pl.kchunk; p.ksplit = pl.ksplit;
   p.nct_pad = pl.nct_pad;
   p.alpha = alpha; p.beta = beta; p.read_c = read_c;
-  bool vec;
-  if (k.form == kFormDot)
-    vec = ((uintptr_t)g.A % 16 == 0) && (g.sAm % 4 == 0 || g.M == 1);
-  else
-    vec = ((uintptr_t)g.A % 16 == 0) && (g.sAk % 4 == 0 || g.K == 1) &&
-          g.M % 4 == 0;
+  const bool vec = ((uintptr_t)g.A % 16 == 0) && (g.sAk % 4 == 0 || g.K == 1) &&
+                   g.M % 4 == 0;
   e = launch_skinny(k.form, v.rt, v.ct, vec, v.wpb, p, st);
   if (e != hipSuccess) return from_hip(e, "skinny kernel launch");
   if (pl.ksplit > 1) {
@@ -499,13 +564,13 @@ hmlStatus hmlSgemmCheckF64(hmlOrder order, hmlOp transA, hmlOp transB,
   e = launch_check_f64(p, &nb, nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
-    hipFree(dstats);
+    (void)hipFree(dstats);
     return from_hip(e, "check kernel");
   }
   double *h = (double *)malloc((size_t)nblk * 6 * sizeof(double));
-  if (!h) { hipFree(dstats); return fail(HML_STATUS_ALLOC_FAILED, "host stats"); }
+  if (!h) { (void)hipFree(dstats); return fail(HML_STATUS_ALLOC_FAILED, "host stats"); }
   e = hipMemcpy(h, dstats, (size_t)nblk * 6 * sizeof(double), hipMemcpyDeviceToHost);
-  hipFree(dstats);
+  (void)hipFree(dstats);
   if (e != hipSuccess) { free(h); return from_hip(e, "copy check stats"); }
   for (int64_t b = 0; b < nblk; ++b) {
     stats[0] += h[b * 6 + 0];

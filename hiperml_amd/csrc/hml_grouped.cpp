@@ -1,7 +1,7 @@
 // hml_grouped.cpp -- grouped (batched) skinny SGEMM: a plan holds the device
-// descriptors of many independent row-major problems, their packed B, and a
-// longest-first tile schedule; running it is two launches (batched B pack +
-// persistent grouped kernel), optionally replayed from a captured hipGraph.
+// descriptors of many independent row-major problems and a longest-first
+// tile schedule; running it is one launch of the persistent grouped kernel,
+// optionally replayed from a captured hipGraph.
 // No reference counterpart (the reference runs one problem per call,
 // src/hml_sgemm.cu:120-211).
 #include <stdio.h>
@@ -18,11 +18,9 @@
 struct hmlGroupedPlan_t {
   int32_t count = 0;
   int64_t ntiles = 0;
-  int64_t max_pack = 0;
   int nblocks = 0;
-  void *dev = nullptr;  // one allocation: problems | packs | tiles | Bp
+  void *dev = nullptr;  // one allocation: problems | tiles
   hml::GroupProblem *probs = nullptr;
-  hml::GroupPack *packs = nullptr;
   hml::GroupTile *tiles = nullptr;
   size_t bytes = 0;
   hipGraph_t graph = nullptr;
@@ -36,10 +34,8 @@ namespace {
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 hmlStatus record(hmlGroupedPlan_t *p, hipStream_t st) {
-  hipError_t e = launch_pack_group(p->packs, p->count, p->max_pack, st);
-  if (e != hipSuccess) return from_hip(e, "grouped pack");
-  e = launch_grouped(p->probs, p->tiles, p->ntiles, p->nblocks, st);
-  return from_hip(e, "grouped kernel");
+  return from_hip(launch_grouped(p->probs, p->tiles, p->ntiles, p->nblocks, st),
+                  "grouped kernel");
 }
 
 }  // namespace
@@ -60,12 +56,8 @@ hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan_t **plan, int32_t count,
     return fail(HML_STATUS_INVALID_VALUE, "grouped: NULL argument or count <= 0");
   if (count > 65535) return fail(HML_STATUS_INVALID_VALUE, "grouped: at most 65535 problems");
   std::vector<GroupProblem> probs(count);
-  std::vector<GroupPack> packs(count);
-  std::vector<size_t> bp_off(count);
   std::vector<int64_t> tile_cost;
   std::vector<GroupTile> tiles;
-  size_t bp_total = 0;
-  int64_t max_pack = 0;
   for (int32_t i = 0; i < count; ++i) {
     if (M[i] <= 0 || N[i] <= 0 || K[i] <= 0)
       return fail(HML_STATUS_INVALID_VALUE, "grouped: problem %d has a zero extent", i);
@@ -79,31 +71,28 @@ hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan_t **plan, int32_t count,
     if (g.swapped || g.form != kFormDot)
       return fail(HML_STATUS_UNSUPPORTED,
                   "grouped: problem %d has N > M (only tall problems are grouped)", i);
+    // the tile body addresses A and B with 31-bit buffer offsets
+    if (((15.0 * lda[i] + K[i] + 4) * 4.0 >= 2147483647.0) ||
+        ((K[i] + 16.0) * ldb[i] + 63.0) * 4.0 >= 2147483647.0)
+      return fail(HML_STATUS_UNSUPPORTED, "grouped: problem %d strides too large", i);
     GemmKey key = make_key(g, beta, 0, HML_ROW_MAJOR, HML_OP_N, HML_OP_N);
     key.gemv_ok = false;
     hmlSgemmVariant v;
     int src;
     select_sgemm(key, &v, &src);
     const int64_t nct = ceil_div(N[i], 16);
-    int ct = (v.family == HML_SGEMM_FAMILY_DOT) ? v.ct : 4;
+    // (the grouped kernel instantiates 1 and 2 column tiles per pass: its
+    // register budget then allows 2 waves per SIMD)
+    int ct = (v.family == HML_SGEMM_FAMILY_DOT) ? v.ct : 2;
     while (ct > 1 && ct / 2 >= nct) ct /= 2;
-    if (ct > 4) ct = 4;
-    const int nct_pad = (int)(ceil_div(nct, ct) * ct);
+    if (ct > 2) ct = 2;
     GroupProblem &p = probs[i];
-    p.A = A[i]; p.lda = lda[i]; p.C = C[i]; p.ldc = ldc[i];
+    p.A = A[i]; p.lda = lda[i]; p.B = B[i]; p.ldb = ldb[i]; p.C = C[i]; p.ldc = ldc[i];
     p.M = M[i]; p.N = N[i]; p.K = K[i];
-    p.nct_pad = nct_pad; p.ct = ct;
-    p.vec = ((uintptr_t)A[i] % 16 == 0 && lda[i] % 4 == 0) ? 1 : 0;
+    p.ct = ct;
+    p.avec = ((uintptr_t)A[i] % 16 == 0 && lda[i] % 4 == 0) ? 1 : 0;
     p.read_c = beta != 0.0f ? 1 : 0;
     p.alpha = alpha; p.beta = beta;
-    const int64_t nsteps = ceil_div(K[i], kDotKStep);
-    const int64_t elems = nsteps * nct_pad * 128;  // f4 fragments
-    bp_off[i] = bp_total;
-    bp_total += align256((size_t)elems * 16);
-    max_pack = std::max(max_pack, elems);
-    GroupPack &q = packs[i];
-    q.B = B[i]; q.sBk = ldb[i]; q.sBn = 1; q.K = K[i]; q.N = N[i];
-    q.nct_pad = nct_pad; q.pad = 0;
     const int64_t nrb = ceil_div(M[i], kDotRowsPerTile);
     // cost of one 16-row tile: A bytes plus C bytes, in 4-byte words
     const int64_t cost = 16 * K[i] + 16 * N[i] * (1 + p.read_c) + 64;
@@ -123,7 +112,6 @@ hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan_t **plan, int32_t count,
   hmlGroupedPlan_t *pl = new hmlGroupedPlan_t();
   pl->count = count;
   pl->ntiles = (int64_t)sorted.size();
-  pl->max_pack = max_pack;
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess) {
     hipDeviceProp_t prop;
@@ -133,31 +121,23 @@ hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan_t **plan, int32_t count,
   // needed to give every wave a tile
   pl->nblocks = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)cus * 2, ceil_div(pl->ntiles, 4)));
   const size_t o_probs = 0;
-  const size_t o_packs = align256(o_probs + sizeof(GroupProblem) * count);
-  const size_t o_tiles = align256(o_packs + sizeof(GroupPack) * count);
-  const size_t o_bp = align256(o_tiles + sizeof(GroupTile) * sorted.size());
-  pl->bytes = o_bp + bp_total;
+  const size_t o_tiles = align256(o_probs + sizeof(GroupProblem) * count);
+  pl->bytes = align256(o_tiles + sizeof(GroupTile) * sorted.size());
   hipError_t e = hipMalloc(&pl->dev, pl->bytes);
   if (e != hipSuccess) {
+    const size_t b = pl->bytes;
     delete pl;
-    return fail(HML_STATUS_ALLOC_FAILED, "grouped plan hipMalloc(%zu): %s",
-                o_bp + bp_total, hipGetErrorString(e));
+    return fail(HML_STATUS_ALLOC_FAILED, "grouped plan hipMalloc(%zu): %s", b,
+                hipGetErrorString(e));
   }
   char *base = (char *)pl->dev;
-  for (int32_t i = 0; i < count; ++i) {
-    probs[i].Bp = (const float *)(base + o_bp + bp_off[i]);
-    packs[i].Bp = (float *)(base + o_bp + bp_off[i]);
-  }
   pl->probs = (GroupProblem *)(base + o_probs);
-  pl->packs = (GroupPack *)(base + o_packs);
   pl->tiles = (GroupTile *)(base + o_tiles);
   e = hipMemcpy(pl->probs, probs.data(), sizeof(GroupProblem) * count, hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = hipMemcpy(pl->packs, packs.data(), sizeof(GroupPack) * count, hipMemcpyHostToDevice);
-  if (e == hipSuccess)
     e = hipMemcpy(pl->tiles, sorted.data(), sizeof(GroupTile) * sorted.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
-    hipFree(pl->dev);
+    (void)hipFree(pl->dev);
     delete pl;
     return from_hip(e, "grouped plan upload");
   }
@@ -181,7 +161,7 @@ hmlStatus hmlSgemmGroupedRun(hmlGroupedPlan_t *plan, int32_t use_graph,
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(plan->graph_stream, &g);
     if (s != HML_STATUS_SUCCESS) {
-      if (g) hipGraphDestroy(g);
+      if (g) (void)hipGraphDestroy(g);
       return s;
     }
     if (e != hipSuccess) return from_hip(e, "hipStreamEndCapture");
@@ -206,11 +186,11 @@ hmlStatus hmlSgemmGroupedInfo(hmlGroupedPlan_t *plan, int64_t *ntiles,
 
 hmlStatus hmlSgemmGroupedDestroy(hmlGroupedPlan_t *plan) {
   if (!plan) return HML_STATUS_SUCCESS;
-  hipDeviceSynchronize();
-  if (plan->exec) hipGraphExecDestroy(plan->exec);
-  if (plan->graph) hipGraphDestroy(plan->graph);
-  if (plan->graph_stream) hipStreamDestroy(plan->graph_stream);
-  if (plan->dev) hipFree(plan->dev);
+  (void)hipDeviceSynchronize();
+  if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+  if (plan->graph) (void)hipGraphDestroy(plan->graph);
+  if (plan->graph_stream) (void)hipStreamDestroy(plan->graph_stream);
+  if (plan->dev) (void)hipFree(plan->dev);
   delete plan;
   return HML_STATUS_SUCCESS;
 }

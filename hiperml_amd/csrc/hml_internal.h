@@ -14,9 +14,8 @@ enum Form : int {
   kFormAxpy = 1,  // A(m,k) at A[m + k*sAk], contiguous along M
 };
 
-// k-step (K values consumed per loop iteration) of each form.  The packed B
-// layout and the K-split granularity are multiples of it.
-constexpr int kDotKStep = 32;
+// k-step (K values consumed per loop iteration) of the AXPY form; its packed
+// B layout and K-split granularity are multiples of it.
 constexpr int kAxpyKStep = 16;
 // Rows covered by one "row tile" of each form.
 constexpr int kDotRowsPerTile = 16;
@@ -50,16 +49,41 @@ struct GemvParams {
   int32_t read_c;
 };
 
-// Packs B(k,n) = B[k*sBk + n*sBn] into fragment order for `form`.
+// Packs B(k,n) = B[k*sBk + n*sBn] into the AXPY kernel's fragment order.
 hipError_t launch_pack_b(const float *B, int64_t sBk, int64_t sBn, int64_t K,
-                         int64_t N, int form, int64_t nsteps, int nct_pad,
-                         float *Bp, hipStream_t st);
-size_t packed_b_floats(int form, int64_t K, int nct_pad);
+                         int64_t N, int64_t nsteps, int nct_pad, float *Bp,
+                         hipStream_t st);
+size_t packed_b_floats(int64_t K, int nct_pad);
 
 // Returns hipErrorInvalidValue when (form, rt, ct) has no instantiation.
 hipError_t launch_skinny(int form, int rt, int ct, bool vec, int wpb,
                          const SkinnyParams &p, hipStream_t st);
 bool skinny_has_variant(int form, int rt, int ct);
+
+// DOT-form skinny SGEMM (hml_k_skinny.hip, k_skinny_dot): B read in place,
+// A staged per wave through LDS, K split over the waves of a workgroup and
+// summed there; kgroups > 1 splits K further over workgroups (partials in P,
+// summed by launch_reduce with R = kgroups).
+constexpr int kDotStageK = 64;  // K per pipeline stage and K-split quantum
+struct DotParams {
+  const float *A;
+  int64_t sAm;                    // A(m,k) = A[m*sAm + k]
+  const float *B;
+  int64_t sBk, sBn;               // B(k,n) = B[k*sBk + n*sBn]
+  float *C;
+  int64_t sCm, sCn;
+  float *P;                       // [kgroups][M*N] when kgroups > 1
+  int64_t M, N, K;
+  int64_t kchunk;                 // K per wave, multiple of kDotStageK
+  int32_t kgroups;
+  int32_t read_c;
+  float alpha, beta;
+};
+// kw = waves per workgroup (each with its own K chunk)
+bool skinny_dot_runnable(int rt, int ct, int kw);
+size_t skinny_dot_lds_bytes(int rt, int ct, int kw);
+hipError_t launch_skinny_dot(int rt, int ct, bool avec, bool bvec, int kw,
+                             const DotParams &p, hipStream_t st);
 
 // out(m,n) = alpha * sum_r P[r*total + m*ninner + n] + beta * out(m,n)
 hipError_t launch_reduce(const float *P, int R, int64_t total, int64_t ninner,
@@ -105,18 +129,18 @@ hipError_t launch_compare(const float *C, const float *R, int64_t M, int64_t N,
 
 namespace hml {
 // One problem of a grouped (batched) skinny SGEMM, row-major NN after
-// canonicalisation (DOT form): C(m,n) = alpha sum_k A[m*lda+k] B(k,n) +
-// beta C[m*ldc+n], B already packed into Bp.
+// canonicalisation (DOT form): C(m,n) = alpha sum_k A[m*lda+k] B[k*ldb+n] +
+// beta C[m*ldc+n].
 struct GroupProblem {
   const float *A;
   int64_t lda;
-  const float *Bp;
+  const float *B;
+  int64_t ldb;
   float *C;
   int64_t ldc;
   int64_t M, N, K;
-  int32_t nct_pad;  // column tiles in Bp (multiple of ct)
-  int32_t ct;       // column tiles per pass: 1, 2 or 4
-  int32_t vec;      // A 16-byte aligned with lda % 4 == 0
+  int32_t ct;       // column tiles per pass: 1 or 2
+  int32_t avec;     // A 16-byte aligned with lda % 4 == 0
   int32_t read_c;
   float alpha, beta;
 };
@@ -125,16 +149,6 @@ struct GroupTile {
   int32_t pad;
   int64_t rb;  // row block of 16 rows
 };
-struct GroupPack {
-  const float *B;
-  int64_t sBk, sBn, K, N;
-  int32_t nct_pad;
-  int32_t pad;
-  float *Bp;
-};
 hipError_t launch_grouped(const GroupProblem *probs, const GroupTile *tiles,
                           int64_t ntiles, int nblocks, hipStream_t st);
-// Packs every problem's B (DOT form) in one launch.
-hipError_t launch_pack_group(const GroupPack *packs, int32_t count,
-                             int64_t max_elems, hipStream_t st);
 }  // namespace hml

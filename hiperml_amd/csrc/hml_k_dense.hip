@@ -118,8 +118,8 @@ __device__ __forceinline__ void store_c(const DenseParams &p, int64_t row,
 }
 
 // ---- split-precision bf16 GEMM ---------------------------------------------
-template <int S>
-__global__ __launch_bounds__(256) void k_dense_bf16(DenseParams p) {
+template <int S, int D>
+__global__ __launch_bounds__(256, (S == 2 ? 2 : 1)) void k_dense_bf16(DenseParams p) {
   __shared__ __attribute__((aligned(16))) uint16_t As[2][S][BM][BKH + PADH];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[2][S][BN][BKH + PADH];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -130,26 +130,29 @@ __global__ __launch_bounds__(256) void k_dense_bf16(DenseParams p) {
   const uint16_t *Bg = (const uint16_t *)p.Bp + bn * BN * p.Kp;
   const int64_t planeA = p.Mp * p.Kp, planeB = p.Np * p.Kp;
 
-  // each thread moves 2 chunks of 8 bf16 per plane per operand per step
-  u4 ra[S][2], rb[S][2];
-  auto gload = [&](int64_t k0) {
+  // Each thread moves 2 chunks of 8 bf16 per plane per operand per step.
+  // Global loads run D steps ahead in D register sets: the set of step
+  // t + D is loaded while step t computes and stored to LDS at the end of
+  // step t + D - 1, so each load has D compute phases to arrive.
+  u4 ra[D][S][2], rb[D][S][2];
+  auto gload = [&](int set, int64_t k0) {
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         const int c = tid + 256 * i, row = c >> 2, kq = c & 3;
-        ra[s][i] = *reinterpret_cast<const u4 *>(Ag + s * planeA + row * p.Kp + k0 + 8 * kq);
-        rb[s][i] = *reinterpret_cast<const u4 *>(Bg + s * planeB + row * p.Kp + k0 + 8 * kq);
+        ra[set][s][i] = *reinterpret_cast<const u4 *>(Ag + s * planeA + row * p.Kp + k0 + 8 * kq);
+        rb[set][s][i] = *reinterpret_cast<const u4 *>(Bg + s * planeB + row * p.Kp + k0 + 8 * kq);
       }
   };
-  auto lstore = [&](int buf) {
+  auto lstore = [&](int set, int buf) {
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         const int c = tid + 256 * i, row = c >> 2, kq = c & 3;
-        *reinterpret_cast<u4 *>(&As[buf][s][row][8 * kq]) = ra[s][i];
-        *reinterpret_cast<u4 *>(&Bs[buf][s][row][8 * kq]) = rb[s][i];
+        *reinterpret_cast<u4 *>(&As[buf][s][row][8 * kq]) = ra[set][s][i];
+        *reinterpret_cast<u4 *>(&Bs[buf][s][row][8 * kq]) = rb[set][s][i];
       }
   };
 
@@ -161,43 +164,48 @@ __global__ __launch_bounds__(256) void k_dense_bf16(DenseParams p) {
 
   const int r16 = lane & 15, g = lane >> 4;
   const int64_t nsteps = p.Kp / BKH;
-  gload(0);
-  lstore(0);
-  __syncthreads();
-  for (int64_t t = 0; t < nsteps; ++t) {
-    const int buf = (int)(t & 1);
-    if (t + 1 < nsteps) gload((t + 1) * BKH);
+  // step t: LDS buffer t % 2, register set of step t + D: (t + D) % D (both
+  // compile-time in the body below, which the loop runs for even and odd t)
+  auto step = [&](int64_t t, int cur) {
+    if (t + D < nsteps) gload(D == 1 ? 0 : cur, (t + D) * BKH);
     bf8 fa[S][4], fb[S][4];
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        fa[s][i] = *reinterpret_cast<const bf8 *>(&As[buf][s][wr * 64 + i * 16 + r16][8 * g]);
-        fb[s][i] = *reinterpret_cast<const bf8 *>(&Bs[buf][s][wc * 64 + i * 16 + r16][8 * g]);
+        fa[s][i] = *reinterpret_cast<const bf8 *>(&As[cur][s][wr * 64 + i * 16 + r16][8 * g]);
+        fb[s][i] = *reinterpret_cast<const bf8 *>(&Bs[cur][s][wc * 64 + i * 16 + r16][8 * g]);
       }
     // products, smallest terms first: (S==3) m*m, l*h, h*l, m*h, h*m, h*h;
-    // (S==2) l*h, h*l, h*h
+    // (S==2) l*h, h*l, h*h.  The product loop is outermost so that
+    // consecutive MFMAs update different tiles (no back-to-back dependency);
+    // each tile still accumulates its products in the order above.
+    constexpr int NP = S == 3 ? 6 : 3;
+    constexpr int PA3[6] = {1, 2, 0, 1, 0, 0}, PB3[6] = {1, 0, 2, 0, 1, 0};
+    constexpr int PA2[3] = {1, 0, 0}, PB2[3] = {0, 1, 0};
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int q = 0; q < NP; ++q) {
+      const int ia = S == 3 ? PA3[q] : PA2[q];
+      const int ib = S == 3 ? PB3[q] : PB2[q];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        f4 c = acc[i][j];
-        if (S == 3) {
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[1][i], fb[1][j], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[2][i], fb[0][j], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0][i], fb[2][j], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[1][i], fb[0][j], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0][i], fb[1][j], c, 0, 0, 0);
-        } else if (S == 2) {
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[1][i], fb[0][j], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0][i], fb[1][j], c, 0, 0, 0);
-        }
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0][i], fb[0][j], c, 0, 0, 0);
-        acc[i][j] = c;
-      }
-    if (t + 1 < nsteps) lstore(buf ^ 1);
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ia][i], fb[ib][j], acc[i][j], 0, 0, 0);
+    }
+    if (t + 1 < nsteps) lstore(D == 1 ? 0 : cur ^ 1, cur ^ 1);
     __syncthreads();
+  };
+  gload(0, 0);
+  lstore(0, 0);
+  if (D == 2 && nsteps > 1) gload(1, BKH);
+  __syncthreads();
+  int64_t t = 0;
+  for (; t + 1 < nsteps; t += 2) {
+    step(t, 0);
+    step(t + 1, 1);
   }
+  if (t < nsteps) step(t, 0);
 
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -293,6 +301,9 @@ hipError_t launch_split(const float *src, int64_t sR, int64_t sK, int64_t rows,
   return hipGetLastError();
 }
 
+// global prefetch depth (steps) of the bf16x3 / bf16x6 kernels
+constexpr int kPrefetch2 = 2, kPrefetch3 = 2;
+
 hipError_t launch_dense(int S, const DenseParams &p, hipStream_t st) {
   const int64_t nblk = (p.Mp / BM) * (p.Np / BN);
   if (nblk <= 0) return hipSuccess;
@@ -300,8 +311,8 @@ hipError_t launch_dense(int S, const DenseParams &p, hipStream_t st) {
   const dim3 grid((unsigned)nblk);
   switch (S) {
     case 0: hipLaunchKernelGGL(k_dense_f32, grid, dim3(256), 0, st, p); break;
-    case 2: hipLaunchKernelGGL(k_dense_bf16<2>, grid, dim3(256), 0, st, p); break;
-    case 3: hipLaunchKernelGGL(k_dense_bf16<3>, grid, dim3(256), 0, st, p); break;
+    case 2: hipLaunchKernelGGL((k_dense_bf16<2, kPrefetch2>), grid, dim3(256), 0, st, p); break;
+    case 3: hipLaunchKernelGGL((k_dense_bf16<3, kPrefetch3>), grid, dim3(256), 0, st, p); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -32,38 +32,6 @@ __device__ __forceinline__ f4 mfma4(float a, float b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// Loads 4 consecutive floats p[0..3] of which the first `nvalid` are in
-// range (0..4); the rest read as zero.  `vec` says p is 16-byte aligned.
-template <bool VEC>
-__device__ __forceinline__ f4 load4(const float *p, int64_t nvalid) {
-  f4 v = {0.f, 0.f, 0.f, 0.f};
-  if (nvalid >= 4) {
-    if (VEC) {
-      v = *reinterpret_cast<const f4 *>(p);
-    } else {
-      v[0] = p[0]; v[1] = p[1]; v[2] = p[2]; v[3] = p[3];
-    }
-  } else if (nvalid > 0) {
-    v[0] = p[0];
-    if (nvalid > 1) v[1] = p[1];
-    if (nvalid > 2) v[2] = p[2];
-  }
-  return v;
-}
-
-// Unguarded load of 4 consecutive floats that are read exactly once
-// (non-temporal, so the streamed operand does not evict B from the caches).
-template <bool VEC>
-__device__ __forceinline__ f4 stream4(const float *p) {
-  if (VEC) return __builtin_nontemporal_load(reinterpret_cast<const f4 *>(p));
-  f4 v;
-  v[0] = __builtin_nontemporal_load(p);
-  v[1] = __builtin_nontemporal_load(p + 1);
-  v[2] = __builtin_nontemporal_load(p + 2);
-  v[3] = __builtin_nontemporal_load(p + 3);
-  return v;
-}
-
 __device__ __forceinline__ void store_out(const SkinnyParams &p, int64_t row,
                                           int64_t col, int ks, float acc) {
   if (row < p.M && col < p.N) {
@@ -86,181 +54,367 @@ __device__ __forceinline__ void store_out(const SkinnyParams &p, int64_t row,
 // checks afterwards.
 
 // ---------------------------------------------------------------------------
-// DOT form: A(m,k) = A[m*sAm + k].  Per k-step of 32, lane (g = l/16,
-// r = l%16) of row tile t loads A(row_t + r, k0 + 16v + 4g .. +3) for
-// v = 0, 1: each 128-bit load instruction reads 64 contiguous bytes of each
-// of its 16 rows.  MFMA number 4v+j consumes element j of load v, so the
-// shared K permutation is k = k0 + 16v + 4g + j.  The packed B fragment of
-// (step s, column tile c, half v) holds, for lane (g, r),
-// B(32s + 16v + 4g + j, 16c + r) for j = 0..3.
+// DOT form, one problem: k_skinny_dot.
+//
+// A wave computes rows [row0, row0 + 16 RT) x column tiles [ct0, ct0 + CT)
+// over its own K chunk; the kw waves of a workgroup take consecutive chunks
+// of the same tile and sum their results through LDS at the end, in wave
+// order, so a K split needs no second kernel unless it spans workgroups.
+//
+// A streams in stages of 64 k.  The global loads of a stage are
+// row-contiguous: load instruction q moves rows row0 + 4q + h (h = l/16) at
+// k = 4(l%16) .. +3, i.e. 4 rows x 256 contiguous bytes -- whole 128-byte
+// lines -- where the MFMA operand order itself would fetch 64 bytes of each
+// of 16 rows per instruction (half lines, twice the line requests per byte).
+// The stage goes into the wave's own LDS tile (16 RT rows x 64 floats, the
+// 16-byte chunk kc of row x stored at chunk kc ^ (x % 16), which spreads
+// both the row-contiguous writes and the operand-order reads over all 64
+// banks) and comes back in operand order: lane (g, r) takes
+// A(row r, k0 + 16v + 4g .. +3), v = 0, 1, per k-step of 32.  The tile is
+// private to the wave, so only the wave's own LDS ordering is needed, not a
+// barrier.
+//
+// B(k, n) is read in place with its own strides: lane
+// (g, r) of column tile c needs B(k0 + 16v + 4g + j, 16c + r), j = 0..3 --
+// one 16-byte load when B is K-contiguous and aligned (BVEC), four scalar
+// loads otherwise.  Columns past N read column N - 1: they feed only output
+// columns that are never stored.  K past the chunk end reads as zero (A and
+// B), so no NaN from memory outside the operands can reach a sum.
+//
+// The A and B loads of a stage are issued together, PS stages ahead, into
+// PS register sets: all vector loads retire through one in-order counter,
+// so waiting for a stage's B must not also wait for later stages' A.
 // ---------------------------------------------------------------------------
-// One wave computes rows [rb*16*RT, (rb+1)*16*RT) x column tiles
-// [ct0, ct0+CT) over the K range of split ks.
-template <int RT, int CT, bool VEC>
-__device__ __forceinline__ void dot_tile(const SkinnyParams &p, int64_t rb,
-                                         int ks, int ct0) {
-  constexpr int P = (RT + CT <= 3) ? 4 : 2;
-  const int lane = threadIdx.x & 63;
-  const int64_t rows_per_wave = (int64_t)kDotRowsPerTile * RT;
-  const int r = lane & 15, g = lane >> 4;
+// pipeline depth (stages of A and B loads in flight) and waves per
+// workgroup of each tile shape, from its register budget.  Two stages per
+// wave measured as fast as three or four on MI355X (8192 x 8192 A, N <= 32)
+// and leave registers for more resident waves.
+__host__ __device__ constexpr int dot_stages(int, int) { return 2; }
+__host__ __device__ constexpr int dot_max_waves(int rt, int ct) {
+  return (rt * ct >= 8 || rt == 4) ? 4 : 8;
+}
+// floats of LDS per wave: the A tile, or the accumulators for the final sum
+__host__ __device__ constexpr int dot_wave_floats(int rt, int ct) {
+  return rt * 16 * kDotStageK > rt * ct * 256 ? rt * 16 * kDotStageK : rt * ct * 256;
+}
+constexpr size_t kDotMaxLds = 65536;
 
-  const float *arow[RT];
+// A tile address (floats) of row x, 16-byte chunk kc
+__device__ __forceinline__ int dot_tile_off(int x, int kc) {
+  return x * kDotStageK + 4 * (kc ^ (x & 15));
+}
+
+// Raw buffer descriptor (gfx9 format word) of `bytes` bytes at `base`.  A
+// load whose lane offset is >= the size returns zero and reads nothing.
+constexpr int kBufNT = 2;                 // cache policy bit: non-temporal
+constexpr int32_t kRsrcMax = 0x7fffffff;  // size of the operand descriptors
+constexpr uint32_t kOOB = 0x80000000u;    // lane offset past every descriptor
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base,
+                                                           int32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, bytes, 0x00020000);
+}
+template <bool VEC, int AUX>
+__device__ __forceinline__ f4 buf_load4(__amdgpu_buffer_rsrc_t rs, uint32_t voff,
+                                        int32_t soff) {
+  if (VEC)
+    return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, AUX));
+  f4 v;
 #pragma unroll
-  for (int t = 0; t < RT; ++t) {
-    int64_t row = rb * rows_per_wave + t * 16 + r;
-    if (row > p.M - 1) row = p.M - 1;  // clamp: loads stay in range, stores are masked
-    arow[t] = p.A + row * p.sAm;
+  for (int j = 0; j < 4; ++j)
+    v[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff + 4 * j, soff, AUX));
+  return v;
+}
+__device__ __forceinline__ float buf_load1(__amdgpu_buffer_rsrc_t rs, uint32_t voff,
+                                           int32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
+}
+
+__device__ __forceinline__ void dot_store(const DotParams &p, int64_t row,
+                                          int64_t col, int kgi, float v) {
+  if (row < p.M && col < p.N) {
+    if (p.kgroups == 1) {
+      float *cp = p.C + row * p.sCm + col * p.sCn;
+      float o = p.alpha * v;
+      if (p.read_c) o = p.alpha * v + p.beta * (*cp);
+      *cp = o;
+    } else {
+      p.P[((int64_t)kgi * p.M + row) * p.N + col] = v;
+    }
   }
-  const int64_t kb = (int64_t)ks * p.kchunk;
-  int64_t ke = kb + p.kchunk;
-  if (ke > p.K) ke = p.K;
-  const int64_t nfull = ke > kb ? (ke - kb) / kDotKStep : 0;
+}
 
-  const f4 *Bp = reinterpret_cast<const f4 *>(p.Bp);
-  const int64_t bstep = (int64_t)p.nct_pad * 128;  // f4 per k-step
-  const f4 *bp = Bp + (kb / kDotKStep) * bstep + (int64_t)ct0 * 128 + lane;
-  const float *ap[RT];
+// One wave: rows [row0, row0 + 16 RT) x columns [ct0*16, ct0*16 + 16 CT)
+// over k in [kb, ke) (kb a multiple of the 64-k stage), accumulated into
+// acc; T is the wave's private LDS tile (16 RT x 64 floats).
+template <int RT, int CT, bool AVEC, bool BVEC>
+__device__ __forceinline__ void dot_tile_lds(const float *__restrict__ A, int64_t sAm,
+                                             const float *__restrict__ B, int64_t sBk,
+                                             int64_t sBn, int64_t M, int64_t N,
+                                             int64_t row0, int ct0, int64_t kb,
+                                             int64_t ke, float *T, f4 (&acc)[RT][CT]) {
+  constexpr int PS = dot_stages(RT, CT);
+  constexpr int NQ = 4 * RT;  // A load instructions per stage
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15, g = lane >> 4;
+  const int64_t nst = ke > kb ? (ke - kb) / kDotStageK : 0;
+
+  // A and B are read with buffer loads: a wave-uniform descriptor and
+  // stage offset, plus a 32-bit byte offset per lane that carries every
+  // lane-dependent part of the address.  A lane offset of kOOB is past any
+  // descriptor's range, so the load returns zero without touching memory:
+  // that is how rows past M, columns past N and k past the chunk end are
+  // read.  Stages past the chunk load through a descriptor of size 0 (also
+  // zero, no traffic), so the pipeline needs no guarded drain.  The host
+  // guarantees every in-range offset of a wave is below 2^31.
+  const __amdgpu_buffer_rsrc_t ra = make_rsrc(A + row0 * sAm + kb, kRsrcMax);
+  const __amdgpu_buffer_rsrc_t rbs =
+      make_rsrc(B + kb * sBk + (int64_t)ct0 * 16 * sBn, kRsrcMax);
+  const __amdgpu_buffer_rsrc_t rnull = make_rsrc(A, 0);
+  uint32_t aoff[NQ];  // row row0 + 4q + g, k = kb + 4r
 #pragma unroll
-  for (int t = 0; t < RT; ++t) ap[t] = arow[t] + kb + 4 * g;
+  for (int q = 0; q < NQ; ++q)
+    aoff[q] = row0 + 4 * q + g < M ? (uint32_t)(((4 * q + g) * sAm + 4 * r) * 4) : kOOB;
+  uint32_t boff[CT];  // column ct0*16 + 16c + r, k = kb + 4g
+#pragma unroll
+  for (int c = 0; c < CT; ++c)
+    boff[c] = (int64_t)(ct0 + c) * 16 + r < N
+                  ? (uint32_t)((4 * g * sBk + (16 * c + r) * sBn) * 4) : kOOB;
 
-  f4 acc[RT][CT];
 #pragma unroll
   for (int t = 0; t < RT; ++t)
 #pragma unroll
     for (int c = 0; c < CT; ++c) acc[t][c] = f4{0.f, 0.f, 0.f, 0.f};
-
-  f4 a[P][RT][2], b[P][CT][2];
-  auto load = [&](int j, int64_t s) {
+  f4 a[PS][NQ];
+  f4 b[PS][CT][2][2];  // [stage set][column tile][k-step u][half v]
+  auto loadA = [&](int j, int64_t s) {
+    const __amdgpu_buffer_rsrc_t rs = s < nst ? ra : rnull;
+    const int32_t so = (int32_t)(s * kDotStageK * 4);
 #pragma unroll
-    for (int t = 0; t < RT; ++t) {
-      a[j][t][0] = stream4<VEC>(ap[t] + s * kDotKStep);
-      a[j][t][1] = stream4<VEC>(ap[t] + s * kDotKStep + 16);
-    }
-#pragma unroll
-    for (int c = 0; c < CT; ++c) {
-      b[j][c][0] = bp[s * bstep + c * 128];
-      b[j][c][1] = bp[s * bstep + c * 128 + 64];
-    }
+    for (int q = 0; q < NQ; ++q) a[j][q] = buf_load4<AVEC, kBufNT>(rs, aoff[q], so);
   };
-  auto comp = [&](int j) {
-#pragma unroll
-    for (int v = 0; v < 2; ++v)
-#pragma unroll
-      for (int jj = 0; jj < 4; ++jj)
-#pragma unroll
-        for (int t = 0; t < RT; ++t)
-#pragma unroll
-          for (int c = 0; c < CT; ++c)
-            acc[t][c] = mfma4(a[j][t][v][jj], b[j][c][v][jj], acc[t][c]);
-  };
-
-  if (nfull >= P) {
-#pragma unroll
-    for (int j = 0; j < P; ++j) {
-      load(j, j);
-      __builtin_amdgcn_sched_barrier(0);  // issue the stages in order
-    }
-    int64_t s = 0;
-    for (; s + 2 * P <= nfull; s += P) {
-#pragma unroll
-      for (int j = 0; j < P; ++j) {
-        comp(j);
-        load(j, s + P + j);
-        __builtin_amdgcn_sched_barrier(0);  // keep each stage's loads right after its compute
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < P; ++j) {
-      comp(j);
-      if (s + P + j < nfull) load(j, s + P + j);
-    }
-#pragma unroll
-    for (int j = 0; j < P; ++j)
-      if (s + P + j < nfull) comp(j);
-  } else {
-    for (int64_t s = 0; s < nfull; ++s) {
-      load(0, s);
-      comp(0);
-    }
-  }
-  if (kb + nfull * kDotKStep < ke) {
-    // partial last step: guarded A, B is zero padded past K
-    const int64_t k0 = kb + nfull * kDotKStep, kk = k0 + 4 * g;
-#pragma unroll
-    for (int t = 0; t < RT; ++t) {
-      a[0][t][0] = load4<VEC>(arow[t] + kk, ke - kk);
-      a[0][t][1] = load4<VEC>(arow[t] + kk + 16, ke - kk - 16);
-    }
-#pragma unroll
-    for (int c = 0; c < CT; ++c) {
-      b[0][c][0] = bp[nfull * bstep + c * 128];
-      b[0][c][1] = bp[nfull * bstep + c * 128 + 64];
-    }
-    comp(0);
-  }
-
-#pragma unroll
-  for (int t = 0; t < RT; ++t)
+  auto loadB = [&](int j, int64_t s) {
+    const __amdgpu_buffer_rsrc_t rs = s < nst ? rbs : rnull;
 #pragma unroll
     for (int c = 0; c < CT; ++c)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int64_t row = rb * rows_per_wave + t * 16 + 4 * g + i;
-        const int64_t col = (int64_t)(ct0 + c) * 16 + r;
-        store_out(p, row, col, ks, acc[t][c][i]);
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+          const int64_t ko = s * kDotStageK + 32 * u + 16 * v;
+          f4 x;
+          if (BVEC) {
+            x = buf_load4<true, 0>(rs, boff[c], (int32_t)(ko * 4));
+          } else {
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj)
+              x[jj] = buf_load1(rs, boff[c], (int32_t)((ko + jj) * sBk * 4));
+          }
+          b[j][c][u][v] = x;
+        }
+  };
+  auto put = [&](int j) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+      *reinterpret_cast<f4 *>(T + dot_tile_off(4 * q + g, r)) = a[j][q];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  auto compute = [&](int j) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      f4 x[RT][2];
+#pragma unroll
+      for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int v = 0; v < 2; ++v)
+          x[t][v] = *reinterpret_cast<const f4 *>(T + dot_tile_off(16 * t + r, 8 * u + 4 * v + g));
+#pragma unroll
+      for (int v = 0; v < 2; ++v)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+          for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int c = 0; c < CT; ++c)
+              acc[t][c] = mfma4(x[t][v][jj], b[j][c][u][v][jj], acc[t][c]);
+    }
+    // the next put() may overwrite the tile only after these reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+
+  // Full stages, PS at a time; stage t uses register set t % PS.  A stage's
+  // A registers are refilled right after they reach LDS, its B registers
+  // after the MFMAs have read them.
+#pragma unroll
+  for (int j = 0; j < PS; ++j) {
+    loadA(j, j);
+    loadB(j, j);
+    __builtin_amdgcn_sched_barrier(0);  // issue the stages in order
+  }
+  for (int64_t s = 0; s < nst; s += PS) {
+#pragma unroll
+    for (int j = 0; j < PS; ++j) {
+      const bool live = s + j < nst;
+      if (live) put(j);
+      loadA(j, s + PS + j);
+      if (live) compute(j);
+      loadB(j, s + PS + j);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+
+  if (kb + nst * kDotStageK < ke) {
+    // partial last stage: elements with k >= ke get the out-of-range offset
+    const int64_t k0 = kb + nst * kDotStageK;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      f4 x;
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) {
+        const int64_t k = k0 + 4 * r + jj;
+        const uint32_t vo = (aoff[q] == kOOB || k >= ke) ? kOOB
+                            : aoff[q] + (uint32_t)((nst * kDotStageK + jj) * 4);
+        x[jj] = buf_load1(ra, vo, 0);
       }
+      a[0][q] = x;
+    }
+#pragma unroll
+    for (int c = 0; c < CT; ++c)
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+          f4 x;
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) {
+            const int64_t ko = nst * kDotStageK + 32 * u + 16 * v + jj;
+            const uint32_t vo = (boff[c] == kOOB || kb + ko + 4 * g >= ke) ? kOOB
+                                : boff[c] + (uint32_t)(ko * sBk * 4);
+            x[jj] = buf_load1(rbs, vo, 0);
+          }
+          b[0][c][u][v] = x;
+        }
+    put(0);
+    compute(0);
+  }
+
 }
 
-template <int RT, int CT, bool VEC>
-__global__ __launch_bounds__(256) void k_skinny_dot(SkinnyParams p) {
-  const int wpb = blockDim.x >> 6;
-  const int64_t task = (int64_t)blockIdx.x * wpb + (threadIdx.x >> 6);
-  const int64_t rows_per_wave = (int64_t)kDotRowsPerTile * RT;
-  const int64_t nrb = (p.M + rows_per_wave - 1) / rows_per_wave;
-  const int64_t rb = task / p.ksplit;
-  const int ks = (int)(task - rb * p.ksplit);
-  if (rb >= nrb) return;  // whole wave leaves; the kernel has no barriers
-  dot_tile<RT, CT, VEC>(p, rb, ks, blockIdx.y * CT);
+template <int RT, int CT, bool AVEC, bool BVEC>
+__global__ __launch_bounds__(64 * dot_max_waves(RT, CT)) void k_skinny_dot(DotParams p) {
+  constexpr int WF = dot_wave_floats(RT, CT);
+  extern __shared__ f4 dot_lds[];
+  // the wave index is uniform across the wave: say so, so that everything
+  // derived from it (K range, descriptors) lives in scalar registers
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int kw = blockDim.x >> 6;
+  const int r = lane & 15, g = lane >> 4;
+  const int64_t rb = blockIdx.x / p.kgroups;
+  const int kgi = (int)(blockIdx.x - rb * p.kgroups);
+  const int64_t row0 = rb * 16 * RT;
+  const int ct0 = blockIdx.y * CT;
+  const int64_t kb = ((int64_t)kgi * kw + w) * p.kchunk;
+  int64_t ke = kb + p.kchunk;
+  if (ke > p.K) ke = p.K;
+  f4 acc[RT][CT];
+  dot_tile_lds<RT, CT, AVEC, BVEC>(p.A, p.sAm, p.B, p.sBk, p.sBn, p.M, p.N, row0,
+                                   ct0, kb, ke, reinterpret_cast<float *>(dot_lds) + w * WF,
+                                   acc);
+
+  if (kw == 1) {
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          dot_store(p, row0 + 16 * t + 4 * g + i, (int64_t)(ct0 + c) * 16 + r, kgi, acc[t][c][i]);
+    return;
+  }
+  // sum over the waves of the workgroup, in wave order (the tiles are done)
+  __syncthreads();
+  constexpr int NT = RT * CT;
+#pragma unroll
+  for (int t = 0; t < RT; ++t)
+#pragma unroll
+    for (int c = 0; c < CT; ++c) dot_lds[(w * NT + t * CT + c) * 64 + lane] = acc[t][c];
+  __syncthreads();
+  for (int q = w; q < NT; q += kw) {
+    f4 sum = dot_lds[q * 64 + lane];
+    for (int ww = 1; ww < kw; ++ww) {
+      const f4 x = dot_lds[(ww * NT + q) * 64 + lane];
+      sum[0] += x[0]; sum[1] += x[1]; sum[2] += x[2]; sum[3] += x[3];
+    }
+    const int t = q / CT, c = q - t * CT;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      dot_store(p, row0 + 16 * t + 4 * g + i, (int64_t)(ct0 + c) * 16 + r, kgi, sum[i]);
+  }
 }
 
 // ---------------------------------------------------------------------------
 // Persistent grouped SGEMM: hundreds of independent skinny problems in one
 // launch.  The host builds one descriptor per problem and a list of
-// (problem, row block) tiles sorted by decreasing cost; wave w of the
+// (problem, 16-row block) tiles sorted by decreasing cost; wave w of the
 // persistent grid takes tiles w, w + W, w + 2W, ... (W = waves launched), a
 // static longest-first schedule that needs no atomics or counters, so the
-// launch can be captured in a graph and replayed.  Each tile runs the same
-// pipelined DOT body as the single-problem kernel (16 rows x every column
-// of its problem, whole K, no split), dispatching on the problem's column
-// tile count and alignment.
+// launch can be captured in a graph and replayed.  Each tile runs the DOT
+// tile body above (B read in place, whole K, no split) over its problem's
+// column tiles, 1 or 2 per pass (more would cost the register budget of a
+// second wave per SIMD), dispatching on that count and A's alignment.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_grouped_dot(const GroupProblem *probs,
                                                      const GroupTile *tiles,
                                                      int64_t ntiles) {
+  __shared__ f4 gtile[4][dot_wave_floats(1, 1) / 4];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int r = lane & 15, g = lane >> 4;
   const int64_t W = (int64_t)gridDim.x * (blockDim.x >> 6);
-  const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
+  float *T = reinterpret_cast<float *>(gtile[wv]);
   for (int64_t t = w; t < ntiles; t += W) {
     const GroupTile tile = tiles[t];
     const GroupProblem &gp = probs[tile.prob];
-    SkinnyParams p{};
-    p.A = gp.A; p.sAm = gp.lda; p.sAk = 1;
-    p.Bp = gp.Bp;
-    p.C = gp.C; p.sCm = gp.ldc; p.sCn = 1;
-    p.P = nullptr;
-    p.M = gp.M; p.N = gp.N; p.K = gp.K;
-    p.kchunk = gp.K > 0 ? gp.K : 1; p.ksplit = 1;
-    p.nct_pad = gp.nct_pad;
-    p.alpha = gp.alpha; p.beta = gp.beta; p.read_c = gp.read_c;
-    const int kind = gp.ct * 2 + gp.vec;
-    for (int ct0 = 0; ct0 < gp.nct_pad; ct0 += gp.ct) {
-      switch (kind) {
-        case 2: dot_tile<1, 1, false>(p, tile.rb, 0, ct0); break;
-        case 3: dot_tile<1, 1, true>(p, tile.rb, 0, ct0); break;
-        case 4: dot_tile<1, 2, false>(p, tile.rb, 0, ct0); break;
-        case 5: dot_tile<1, 2, true>(p, tile.rb, 0, ct0); break;
-        case 8: dot_tile<1, 4, false>(p, tile.rb, 0, ct0); break;
-        case 9: dot_tile<1, 4, true>(p, tile.rb, 0, ct0); break;
-        default: break;
+    const int64_t row0 = tile.rb * 16;
+    const int64_t nct = (gp.N + 15) / 16;
+    for (int ct0 = 0; ct0 < nct; ct0 += gp.ct) {
+      f4 acc[1][2];
+      switch (gp.ct * 2 + gp.avec) {
+#define HML_GROUP_CASE(CT, AV)                                                        \
+  case CT * 2 + AV: {                                                                \
+    f4 a1[1][CT];                                                                    \
+    dot_tile_lds<1, CT, AV, false>(gp.A, gp.lda, gp.B, gp.ldb, 1, gp.M, gp.N, row0,  \
+                                   ct0, 0, gp.K, T, a1);                             \
+    _Pragma("unroll") for (int c = 0; c < CT; ++c) acc[0][c] = a1[0][c];             \
+    break;                                                                           \
+  }
+        HML_GROUP_CASE(1, 0)
+        HML_GROUP_CASE(1, 1)
+        HML_GROUP_CASE(2, 0)
+        HML_GROUP_CASE(2, 1)
+#undef HML_GROUP_CASE
+        default:
+          continue;
+      }
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        if (c >= gp.ct) break;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int64_t row = row0 + 4 * g + i;
+          const int64_t col = (int64_t)(ct0 + c) * 16 + r;
+          if (row < gp.M && col < gp.N) {
+            float *cp = gp.C + row * gp.ldc + col;
+            float v = gp.alpha * acc[0][c][i];
+            if (gp.read_c) v = gp.alpha * acc[0][c][i] + gp.beta * (*cp);
+            *cp = v;
+          }
+        }
       }
     }
   }
@@ -418,58 +572,29 @@ __global__ __launch_bounds__(256) void k_skinny_axpy(SkinnyParams p) {
 }
 
 // ---------------------------------------------------------------------------
-// B packing.  One thread writes one 128-bit fragment element.
+// B packing for the AXPY form.  One thread writes one 128-bit fragment
+// element: lane (g, r) of (k-step s, column tile c) holds B(16s + 4u + g,
+// 16c + r), u = 0..3, zero past K and N.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ f4 pack_elem(const float *__restrict__ B,
-                                        int64_t sBk, int64_t sBn, int64_t K,
-                                        int64_t N, int form, int nct_pad,
-                                        int64_t idx) {
+__global__ void k_pack_b(const float *__restrict__ B, int64_t sBk, int64_t sBn,
+                         int64_t K, int64_t N, int64_t total, int nct_pad,
+                         f4 *__restrict__ Bp) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
   const int lane = (int)(idx & 63);
   const int r = lane & 15, g = lane >> 4;
+  const int64_t sc = idx >> 6;  // s*nct_pad + c
+  const int64_t c = sc % nct_pad, s = sc / nct_pad;
+  const int64_t n = c * 16 + r;
   f4 v = {0.f, 0.f, 0.f, 0.f};
-  int64_t n, kbase, kstride;
-  if (form == kFormDot) {
-    const int64_t q = idx >> 6;         // ((s*nct_pad + c)*2 + half)
-    const int half = (int)(q & 1);
-    const int64_t sc = q >> 1;
-    const int64_t c = sc % nct_pad, s = sc / nct_pad;
-    n = c * 16 + r;
-    kbase = s * kDotKStep + 16 * half + 4 * g;
-    kstride = 1;
-  } else {
-    const int64_t sc = idx >> 6;        // s*nct_pad + c
-    const int64_t c = sc % nct_pad, s = sc / nct_pad;
-    n = c * 16 + r;
-    kbase = s * kAxpyKStep + g;
-    kstride = 4;
-  }
   if (n < N) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int64_t k = kbase + j * kstride;
+      const int64_t k = s * kAxpyKStep + g + 4 * j;
       if (k < K) v[j] = B[k * sBk + n * sBn];
     }
   }
-  return v;
-}
-
-__global__ void k_pack_b(const float *__restrict__ B, int64_t sBk, int64_t sBn,
-                         int64_t K, int64_t N, int form, int64_t total,
-                         int nct_pad, f4 *__restrict__ Bp) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= total) return;
-  Bp[idx] = pack_elem(B, sBk, sBn, K, N, form, nct_pad, idx);
-}
-
-// blockIdx.y = problem; blocks past a problem's packed size exit.
-__global__ void k_pack_group(const GroupPack *__restrict__ packs) {
-  const GroupPack pk = packs[blockIdx.y];
-  const int64_t nsteps = (pk.K + kDotKStep - 1) / kDotKStep;
-  const int64_t total = nsteps * pk.nct_pad * 128;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= total) return;
-  reinterpret_cast<f4 *>(pk.Bp)[idx] =
-      pack_elem(pk.B, pk.sBk, pk.sBn, pk.K, pk.N, kFormDot, pk.nct_pad, idx);
+  Bp[idx] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -543,16 +668,40 @@ __global__ void k_fill(float *dst, size_t n, float value) {
   for (; i < n; i += stride) dst[i] = value;
 }
 
-template <int RT, int CT, bool VEC>
-hipError_t run_dot(int wpb, const SkinnyParams &p, hipStream_t st) {
-  const int64_t rows = (int64_t)kDotRowsPerTile * RT;
-  const int64_t tasks = ((p.M + rows - 1) / rows) * p.ksplit;
-  const int64_t nblk = (tasks + wpb - 1) / wpb;
-  const int64_t ncb = p.nct_pad / CT;
+template <int RT, int CT, bool AV, bool BV>
+hipError_t run_dot(int kw, const DotParams &p, hipStream_t st) {
+  const int64_t nrb = (p.M + 16 * RT - 1) / (16 * RT);
+  const int64_t nct = (p.N + 15) / 16;
+  const int64_t ncb = (nct + CT - 1) / CT;
+  const int64_t nblk = nrb * p.kgroups;
   if (nblk > 0x7fffffff || ncb > 65535) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL((k_skinny_dot<RT, CT, VEC>), dim3((unsigned)nblk, (unsigned)ncb),
-                     dim3(64 * wpb), 0, st, p);
+  const size_t lds = (size_t)kw * dot_wave_floats(RT, CT) * sizeof(float);
+  hipLaunchKernelGGL((k_skinny_dot<RT, CT, AV, BV>), dim3((unsigned)nblk, (unsigned)ncb),
+                     dim3(64 * kw), lds, st, p);
   return hipGetLastError();
+}
+
+typedef hipError_t (*DotFn)(int, const DotParams &, hipStream_t);
+
+template <int RT, int CT>
+DotFn dot_fn(bool av, bool bv) {
+  return av ? (bv ? run_dot<RT, CT, true, true> : run_dot<RT, CT, true, false>)
+            : (bv ? run_dot<RT, CT, false, true> : run_dot<RT, CT, false, false>);
+}
+
+// instantiated (rt, ct): ct <= 4, rt * ct <= 8
+DotFn find_dot(int rt, int ct, bool av, bool bv) {
+  switch (rt * 16 + ct) {
+    case 17: return dot_fn<1, 1>(av, bv);
+    case 18: return dot_fn<1, 2>(av, bv);
+    case 20: return dot_fn<1, 4>(av, bv);
+    case 33: return dot_fn<2, 1>(av, bv);
+    case 34: return dot_fn<2, 2>(av, bv);
+    case 36: return dot_fn<2, 4>(av, bv);
+    case 65: return dot_fn<4, 1>(av, bv);
+    case 66: return dot_fn<4, 2>(av, bv);
+    default: return nullptr;
+  }
 }
 
 template <int RT, int CT, bool VEC>
@@ -570,8 +719,6 @@ hipError_t run_axpy(int wpb, const SkinnyParams &p, hipStream_t st) {
 typedef hipError_t (*RunFn)(int, const SkinnyParams &, hipStream_t);
 
 template <int RT, int CT>
-constexpr RunFn dot_fn(bool vec) { return vec ? run_dot<RT, CT, true> : run_dot<RT, CT, false>; }
-template <int RT, int CT>
 constexpr RunFn axpy_fn(bool vec) { return vec ? run_axpy<RT, CT, true> : run_axpy<RT, CT, false>; }
 
 RunFn find_skinny(int form, int rt, int ct, bool vec) {
@@ -584,12 +731,7 @@ RunFn find_skinny(int form, int rt, int ct, bool vec) {
     default: return nullptr;                    \
   }
   if (form == kFormDot) {
-    switch (rt) {
-      case 1: HML_CT_CASES(dot_fn, 1)
-      case 2: HML_CT_CASES(dot_fn, 2)
-      case 4: HML_CT_CASES(dot_fn, 4)
-      default: return nullptr;
-    }
+    return nullptr;  // k_skinny_dot: launch_skinny_dot
   } else if (form == kFormAxpy) {
     switch (rt) {
       case 1: HML_CT_CASES(axpy_fn, 1)
@@ -606,7 +748,25 @@ RunFn find_skinny(int form, int rt, int ct, bool vec) {
 }  // namespace
 
 bool skinny_has_variant(int form, int rt, int ct) {
+  if (form == kFormDot) return find_dot(rt, ct, true, true) != nullptr;
   return find_skinny(form, rt, ct, true) != nullptr;
+}
+
+size_t skinny_dot_lds_bytes(int rt, int ct, int kw) {
+  return (size_t)kw * dot_wave_floats(rt, ct) * sizeof(float);
+}
+
+bool skinny_dot_runnable(int rt, int ct, int kw) {
+  if (!find_dot(rt, ct, true, true)) return false;
+  return kw >= 1 && kw <= dot_max_waves(rt, ct) && skinny_dot_lds_bytes(rt, ct, kw) <= kDotMaxLds;
+}
+
+hipError_t launch_skinny_dot(int rt, int ct, bool avec, bool bvec, int kw,
+                             const DotParams &p, hipStream_t st) {
+  if (!skinny_dot_runnable(rt, ct, kw) || p.kgroups < 1 || p.kchunk % kDotStageK != 0)
+    return hipErrorInvalidValue;
+  DotFn fn = find_dot(rt, ct, avec, bvec);
+  return fn(kw, p, st);
 }
 
 hipError_t launch_skinny(int form, int rt, int ct, bool vec, int wpb,
@@ -617,22 +777,19 @@ hipError_t launch_skinny(int form, int rt, int ct, bool vec, int wpb,
   return fn(wpb, p, st);
 }
 
-size_t packed_b_floats(int form, int64_t K, int nct_pad) {
-  const int64_t step = form == kFormDot ? kDotKStep : kAxpyKStep;
-  const int64_t nsteps = (K + step - 1) / step;
-  // DOT: 2 fragments of 4 floats per lane per (step, tile); AXPY: 1.
-  return (size_t)nsteps * nct_pad * 64 * 4 * (form == kFormDot ? 2 : 1);
+size_t packed_b_floats(int64_t K, int nct_pad) {
+  const int64_t nsteps = (K + kAxpyKStep - 1) / kAxpyKStep;
+  return (size_t)nsteps * nct_pad * 64 * 4;
 }
 
 hipError_t launch_pack_b(const float *B, int64_t sBk, int64_t sBn, int64_t K,
-                         int64_t N, int form, int64_t nsteps, int nct_pad,
-                         float *Bp, hipStream_t st) {
-  const int64_t total = nsteps * nct_pad * 64 * (form == kFormDot ? 2 : 1);
+                         int64_t N, int64_t nsteps, int nct_pad, float *Bp,
+                         hipStream_t st) {
+  const int64_t total = nsteps * nct_pad * 64;
   if (total <= 0) return hipSuccess;
   const int64_t nblk = (total + 255) / 256;
   hipLaunchKernelGGL(k_pack_b, dim3((unsigned)nblk), dim3(256), 0, st, B, sBk,
-                     sBn, K, N, form, total, nct_pad,
-                     reinterpret_cast<f4 *>(Bp));
+                     sBn, K, N, total, nct_pad, reinterpret_cast<f4 *>(Bp));
   return hipGetLastError();
 }
 
@@ -684,14 +841,5 @@ hipError_t launch_grouped(const GroupProblem *probs, const GroupTile *tiles,
   return hipGetLastError();
 }
 
-hipError_t launch_pack_group(const GroupPack *packs, int32_t count,
-                             int64_t max_elems, hipStream_t st) {
-  if (count <= 0 || max_elems <= 0) return hipSuccess;
-  const int64_t nblk = (max_elems + 255) / 256;
-  if (nblk > 0x7fffffff || count > 65535) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL(k_pack_group, dim3((unsigned)nblk, (unsigned)count),
-                     dim3(256), 0, st, packs);
-  return hipGetLastError();
-}
 
 }  // namespace hml

@@ -7,13 +7,16 @@
 // is installed; the config table and the built-in heuristic stay as
 // fallbacks (a pick that cannot run the shape is never used).  Decisions
 // are memoised per problem key, so the network runs once per new shape.
+#include <dlfcn.h>
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
 #include <mutex>
 #include <numeric>
+#include <string>
 #include <unordered_map>
 #include <vector>
 
@@ -50,12 +53,12 @@ uint64_t splitmix(uint64_t *s) {
 double u01(uint64_t *s) { return (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0); }
 
 void free_batch(hmlMlp_t *n) {
-  for (size_t l = 0; l < n->act.size(); ++l) if (n->act[l]) hipFree(n->act[l]);
-  for (size_t l = 0; l < n->dz.size(); ++l) if (n->dz[l]) hipFree(n->dz[l]);
+  for (size_t l = 0; l < n->act.size(); ++l) if (n->act[l]) (void)hipFree(n->act[l]);
+  for (size_t l = 0; l < n->dz.size(); ++l) if (n->dz[l]) (void)hipFree(n->dz[l]);
   n->act.assign(n->dims.size(), nullptr);
   n->dz.assign(n->dims.size(), nullptr);
   for (float **p : {&n->targets, &n->mask, &n->probs, &n->loss_row, &n->loss})
-    if (*p) { hipFree(*p); *p = nullptr; }
+    if (*p) { (void)hipFree(*p); *p = nullptr; }
   n->cap = 0;
 }
 
@@ -169,13 +172,15 @@ bool selector_variant(const GemmKey &k, int cls, hmlSgemmVariant *v) {
   const int ks_i = (cls >> 1) % 7;
   const int rest = (cls >> 1) / 7;  // rt_i * 2 + ctm
   const int ctm = rest & 1, rt_i = rest >> 1;
-  const int ctmax = ctmax_for(k.N);
+  // DOT tiles cover at most 4 column tiles (more columns take more
+  // workgroups); AXPY tiles up to 8
+  const int ctmax = std::min(ctmax_for(k.N), k.form == kFormDot ? 4 : 8);
   if (ctm == 1 && ctmax == 1) return false;
   v->family = k.form == kFormDot ? HML_SGEMM_FAMILY_DOT : HML_SGEMM_FAMILY_AXPY;
   v->rt = 1 << rt_i;
   v->ct = ctm ? ctmax / 2 : ctmax;
   v->ksplit = 1 << ks_i;
-  v->wpb = wpb_i ? 4 : 2;
+  v->wpb = wpb_i ? 8 : 4;
   if (v->rt * v->ct > 16) return false;
   const int64_t cap = std::max<int64_t>(1, std::min<int64_t>(64, k.K / 128));
   if (v->ksplit > cap) return false;
@@ -192,7 +197,54 @@ int selector_class_of(const GemmKey &k, const hmlSgemmVariant &v) {
   return -1;
 }
 
+// The shipped selector: $HML_SELECTOR (a model file, or "off"), else
+// config/hml_selector_mi355x.bin beside the hiperml_amd/ directory that
+// holds this library.  Loaded once per process, on the first selection (on
+// the device current then), and installed unless the caller installed a
+// network first; a missing or unreadable file leaves the table / heuristic
+// selection in place.
+std::once_flag g_autoload_once;
+
+void selector_autoload() {
+  std::call_once(g_autoload_once, [] {
+    const char *env = getenv("HML_SELECTOR");
+    if (env && (!strcmp(env, "off") || !strcmp(env, "0"))) return;
+    std::string path;
+    if (env && *env) {
+      path = env;
+    } else {
+      Dl_info info;
+      if (!dladdr((void *)&hmlMlpLoad, &info) || !info.dli_fname) return;
+      std::string lib = info.dli_fname;
+      const size_t sl = lib.rfind('/');
+      path = (sl == std::string::npos ? std::string(".") : lib.substr(0, sl)) +
+             "/../config/hml_selector_mi355x.bin";
+    }
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) return;
+    fclose(f);
+    hmlMlp_t *net = nullptr;
+    if (hmlMlpLoad(&net, path.c_str()) != HML_STATUS_SUCCESS) {
+      if (verbosity() >= 1)
+        fprintf(stderr, "Warning: selector '%s' not loaded: %s\n", path.c_str(), hmlLastError());
+      clear_error();
+      return;
+    }
+    bool installed = false;
+    {
+      std::lock_guard<std::mutex> lk(g_sel_mutex);
+      if (!g_sel_net && net->n_in == kSelFeatures && net->n_out == kSelClasses) {
+        g_sel_net = net;  // owned by the library until the process ends
+        g_sel_memo.clear();
+        installed = true;
+      }
+    }
+    if (!installed) hmlMlpDestroy(net);
+  });
+}
+
 bool learned_select_sgemm(const GemmKey &k, hmlSgemmVariant *v) {
+  selector_autoload();
   std::lock_guard<std::mutex> lk(g_sel_mutex);
   if (!g_sel_net) return false;
   char key[160];
@@ -277,10 +329,10 @@ hmlStatus hmlMlpDestroy(hmlMlp_t *n) {
     std::lock_guard<std::mutex> lk(g_sel_mutex);
     if (g_sel_net == n) { g_sel_net = nullptr; g_sel_memo.clear(); }
   }
-  if (n->st) hipStreamSynchronize(n->st);
+  if (n->st) (void)hipStreamSynchronize(n->st);
   free_batch(n);
-  for (float *p : {n->params, n->grads, n->adam_m, n->adam_v}) if (p) hipFree(p);
-  if (n->st) hipStreamDestroy(n->st);
+  for (float *p : {n->params, n->grads, n->adam_m, n->adam_v}) if (p) (void)hipFree(p);
+  if (n->st) (void)hipStreamDestroy(n->st);
   delete n;
   return HML_STATUS_SUCCESS;
 }
@@ -289,13 +341,15 @@ int64_t hmlMlpParamCount(hmlMlp_t *n) { return n ? n->nparam : -1; }
 
 hmlStatus hmlMlpGetParams(hmlMlp_t *n, float *host, int64_t count) {
   if (!n || !host || count != n->nparam) return fail(HML_STATUS_INVALID_VALUE, "mlp params");
-  hipStreamSynchronize(n->st);
+  hipError_t e = hipStreamSynchronize(n->st);
+  if (e != hipSuccess) return from_hip(e, "mlp get");
   return from_hip(hipMemcpy(host, n->params, sizeof(float) * count, hipMemcpyDeviceToHost), "mlp get");
 }
 
 hmlStatus hmlMlpSetParams(hmlMlp_t *n, const float *host, int64_t count) {
   if (!n || !host || count != n->nparam) return fail(HML_STATUS_INVALID_VALUE, "mlp params");
-  hipStreamSynchronize(n->st);
+  hipError_t e = hipStreamSynchronize(n->st);
+  if (e != hipSuccess) return from_hip(e, "mlp set");
   {
     std::lock_guard<std::mutex> lk(g_sel_mutex);
     if (g_sel_net == n) g_sel_memo.clear();
@@ -374,7 +428,7 @@ hmlStatus hmlMlpTrain(hmlMlp_t *n, const float *X, const float *T,
       epoch_loss[ep] = (float)(tot / (double)count);
     }
   }
-  for (void *p : {(void *)dX, (void *)dT, (void *)dM, (void *)dIdx, (void *)dL}) if (p) hipFree(p);
+  for (void *p : {(void *)dX, (void *)dT, (void *)dM, (void *)dIdx, (void *)dL}) if (p) (void)hipFree(p);
   {
     std::lock_guard<std::mutex> lk(g_sel_mutex);
     if (g_sel_net == n) g_sel_memo.clear();

@@ -61,7 +61,8 @@ def autotune(shapes, db_path, trials=3, flush=True, seed=1, progress=None):
                                   db_path.encode())
         ok += st == 0
         if progress and (i + 1) % progress == 0:
-            print("autotuned %d/%d shapes" % (i + 1, len(shapes)), flush=True)
+            print("autotuned %d/%d shapes (last %dx%dx%d)" % (i + 1, len(shapes), M, N, K),
+                  flush=True)
     return ok
 
 
@@ -189,12 +190,13 @@ def run(out_dir, n_train, n_eval, seed=2024, epochs=300, trials=3, mem_cap=8e9,
     if train_db is None:
         train_db = os.path.join(out_dir, "autotune_train.csv")
         shapes = random_shapes(n_train, seed, mem_cap)
-        autotune(shapes, train_db, trials=trials, seed=seed, progress=25)
+        autotune(shapes, train_db, trials=trials, seed=seed, progress=max(1, min(25, n_train // 10)))
         print("train autotune done in %.0f s" % (time.time() - t0), flush=True)
     if eval_db is None:
         eval_db = os.path.join(out_dir, "autotune_eval.csv")
         shapes = random_shapes(n_eval, seed + 1, mem_cap)
-        autotune(shapes, eval_db, trials=trials, seed=seed + 100000, progress=50)
+        autotune(shapes, eval_db, trials=trials, seed=seed + 100000,
+                 progress=max(1, min(50, n_eval // 10)))
         print("eval autotune done in %.0f s" % (time.time() - t0), flush=True)
     dbt, dbe = load_db(train_db), load_db(eval_db)
     keys, X, T, mask, _ = dataset(dbt)

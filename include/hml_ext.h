@@ -57,12 +57,19 @@ typedef enum {
   HML_SGEMM_FAMILY_BF16X6 = 6  /* split-precision bf16 MFMA, 3-term split  */
 } hmlSgemmFamily;
 
+/* Skinny variants (DOT, AXPY): a wave computes rt row tiles (16 rows DOT,
+ * 64 rows AXPY) x ct column tiles of 16 over one of ksplit K chunks.  DOT:
+ * the wpb (1, 2, 4 or 8) waves of a workgroup take consecutive chunks of one
+ * tile and sum them in LDS; chunks beyond one workgroup are summed by a
+ * reduction pass; ct <= 4 and rt * ct <= 8.  AXPY: wpb (1, 2 or 4) waves per
+ * workgroup take different tiles; every K split is summed by a reduction
+ * pass. */
 typedef struct {
   int32_t family; /* hmlSgemmFamily                                          */
-  int32_t rt;     /* row tiles per wave (16 rows DOT, 64 rows AXPY)           */
+  int32_t rt;     /* row tiles per wave                                       */
   int32_t ct;     /* column tiles of 16 per wave                              */
-  int32_t ksplit; /* K split across workgroups (partials + reduce), >= 1      */
-  int32_t wpb;    /* waves per workgroup (1, 2 or 4)                          */
+  int32_t ksplit; /* K chunks per tile, >= 1                                  */
+  int32_t wpb;    /* waves per workgroup                                      */
 } hmlSgemmVariant;
 
 typedef enum {
@@ -133,13 +140,14 @@ size_t hmlSgemmDenseWorkspaceSize(int32_t precision, int64_t M, int64_t N,
 
 /* Grouped skinny SGEMM, no reference counterpart: C_i = alpha * A_i * B_i +
  * beta * C_i for `count` independent ROW-MAJOR problems (M_i >= N_i), given
- * as host arrays of device pointers and extents.  The plan owns packed B
- * storage, the device descriptors and a longest-first tile schedule for one
- * persistent kernel (column tiles per problem from the selector).  Run
- * packs every B_i and computes every C_i in two launches; with use_graph the
- * pair is captured into a hipGraph on the first run and replayed after.
- * The A/B/C pointers are captured at creation: rerun after changing their
- * contents, create a new plan for new pointers. */
+ * as host arrays of device pointers and extents.  The plan owns the device
+ * descriptors and a longest-first tile schedule for one persistent kernel
+ * (column tiles per problem from the selector).  Run computes every C_i in
+ * one launch; with use_graph that launch is captured into a hipGraph on the
+ * first run and replayed after.  The A/B/C pointers are captured at
+ * creation: rerun after changing their contents, create a new plan for new
+ * pointers.  HML_STATUS_UNSUPPORTED for N_i > M_i, or strides so large that
+ * 15 * lda_i + K_i or (K_i + 16) * ldb_i reaches 2^29 elements. */
 typedef struct hmlGroupedPlan_t *hmlGroupedPlan;
 hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan *plan, int32_t count,
                                 const int64_t *M, const int64_t *N,

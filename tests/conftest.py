@@ -4,6 +4,10 @@ import sys
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+# Selection tests expect the table / heuristic selectors: keep the shipped
+# learned selector out of this process (tests/test_gpu_mlp.py checks it in a
+# child process of its own).
+os.environ["HML_SELECTOR"] = "off"
 for p in (ROOT, os.path.join(ROOT, "oracle")):
     if p not in sys.path:
         sys.path.insert(0, p)

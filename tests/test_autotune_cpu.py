@@ -58,17 +58,17 @@ def test_candidates_cover_heuristic_and_are_runnable():
 def test_argmin_wins_and_db_records_all(tmp_path):
     s = hml.GemmShape(8192, 16, 8192, 0, 0, 0, 1)
     cands = candidates(s)
-    # fake cost: prefers rt=2, ct=1, ksplit=4, wpb=2; one candidate "fails"
+    # fake cost: prefers rt=2, ct=1, ksplit=4, wpb=8; one candidate "fails"
     def cost(kind, d, v):
         fam, rt, ct, ks, wpb = v
         if (rt, ks, wpb) == (4, 1, 4):
             return -1.0
-        return 1.0 + abs(rt - 2) + abs(ct - 1) + abs(ks - 4) * 0.1 + (wpb != 2) * 0.01
+        return 1.0 + abs(rt - 2) + abs(ct - 1) + abs(ks - 4) * 0.1 + (wpb != 8) * 0.01
     db = str(tmp_path / "db.csv")
     with FakeTimer(cost) as ft:
         st, best, bt, ht = sgemm_autotune(s, db)
     assert st == 0
-    assert best.astuple() == (1, 2, 1, 4, 2)
+    assert best.astuple() == (1, 2, 1, 4, 8)
     assert bt == pytest.approx(1.0)
     assert len(ft.calls) == len(cands)
     rows = list(csv.DictReader(open(db)))

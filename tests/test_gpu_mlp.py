@@ -116,3 +116,25 @@ def test_selector_install_and_fallback(hml, tmp_path):
         hml.selector_install(None)
     v, src = hml.sgemm_select(M, N, K)
     assert src == 0
+
+
+def test_shipped_selector_autoload(hml, tmp_path):
+    """With HML_SELECTOR naming a model file, the first selection in a fresh
+    process installs it (source 2 = learned); HML_SELECTOR=off keeps the
+    heuristic.  Runs in child processes so this process keeps its state."""
+    import subprocess
+    import sys
+    import os
+    net = hml.Mlp(seed=5)
+    path = str(tmp_path / "sel.bin")
+    net.save(path)
+    code = ("import sys; sys.path.insert(0, %r); import hiperml_amd as h; "
+            "h.check(h.lib.hmlInit()); v, s = h.sgemm_select(1 << 16, 16, 4096); "
+            "print('SRC', s)" % os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    for setting, want in ((path, 2), ("off", 0)):
+        env = dict(os.environ, HML_SELECTOR=setting)
+        r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
+                           text=True, timeout=120)
+        print(r.stdout, r.stderr[-1000:])
+        assert r.returncode == 0
+        assert ("SRC %d" % want) in r.stdout

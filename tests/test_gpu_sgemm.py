@@ -260,3 +260,67 @@ def test_full_size_vs_fp64_check(hml, shape):
     assert st["nonfinite"] == 0
     assert st["max_rel_bound"] <= 2 * gamma(K + 2)
     assert st["rel_fro"] <= gamma(K + 2) * st["bound_fro"]
+
+
+def padded(S, ld):
+    """Row-major storage S (rows x cols) copied into rows of length ld."""
+    rows, cols = S.shape
+    P = np.full((rows, ld), np.float32(np.nan))  # padding must never be read
+    P[:, :cols] = S
+    return P
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=lambda l: "%s%s%s" % (l[0], "T" if l[1] else "N", "T" if l[2] else "N"))
+def test_padded_leading_dims(hml, layout):
+    """Leading dimensions larger than the stored extent (NaN in the padding,
+    so any read of it shows up), odd paddings that break 16-byte alignment,
+    integer data: bit-exact."""
+    order, ta, tb = layout
+    rng = np.random.default_rng(5)
+    for (M, N, K, pad) in ((3000, 16, 700, 3), (777, 40, 2100, 37), (64, 5, 64, 1), (20000, 2, 300, 64)):
+        print("case", M, N, K, pad, layout, flush=True)
+        A = int_matrix(rng, (M, K))
+        B = int_matrix(rng, (K, N))
+        C0 = int_matrix(rng, (M, N))
+        As, lda = store(A, order, ta)
+        Bs, ldb = store(B, order, tb)
+        Cs, ldc = store_c(C0, order)
+        lda, ldb, ldc = lda + pad, ldb + pad, ldc + pad
+        dA = hml.DeviceBuffer.from_numpy(padded(As, lda))
+        dB = hml.DeviceBuffer.from_numpy(padded(Bs, ldb))
+        Cp = padded(Cs, ldc)
+        dC = hml.DeviceBuffer.from_numpy(Cp)
+        hml.sgemm_ex(M, N, K, dA, dB, dC, 1.0, 1.0, order=order, transA=ta,
+                     transB=tb, lda=lda, ldb=ldb, ldc=ldc)
+        hml.synchronize()
+        out = dC.to_numpy(Cp.shape)
+        got = out[:, :Cs.shape[1]]
+        if order == "col":
+            got = got.T
+        np.testing.assert_array_equal(got, exact_product(A, B, 1.0, 1.0, C0))
+        # the padding of C is untouched
+        assert np.isnan(out[:, Cs.shape[1]:]).all()
+
+
+def test_large_stride_offsets(hml):
+    """A row stride near the 31-bit byte-offset limit of the skinny kernel's
+    buffer addressing (2 rows, lda = 2^27), and one past it for a full row
+    tile (17 rows, lda = 2^26: the dense kernel takes over).  Integer data,
+    bit-exact; only the valid rows are uploaded."""
+    rng = np.random.default_rng(9)
+    for (M, N, K, lda) in ((2, 2, 100, 1 << 27), (17, 16, 96, 1 << 26)):
+        print("case", M, N, K, lda, flush=True)
+        A = int_matrix(rng, (M, K))
+        B = int_matrix(rng, (K, N))
+        nfl = (M - 1) * lda + K
+        dA = hml.DeviceBuffer(4 * nfl)
+        hml.check(hml.lib.hmlMemsetD8(dA.ptr, 0, 4 * nfl))
+        for i in range(M):
+            row = np.ascontiguousarray(A[i])
+            hml.check(hml.lib.hmlMemcpyHtoD(dA.ptr + 4 * i * lda, row.ctypes.data, row.nbytes))
+        dB = hml.DeviceBuffer.from_numpy(B)
+        dC = hml.DeviceBuffer(4 * M * N)
+        hml.sgemm_ex(M, N, K, dA, dB, dC, lda=lda)
+        hml.synchronize()
+        np.testing.assert_array_equal(dC.to_numpy((M, N)), exact_product(A, B))
+        dA.free()
