@@ -192,6 +192,8 @@ def main():
         except Exception:
             pass
 
+    probe = bandwidth_probe(hml, bufs["gemmA_0"], S)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(hml, calls, args.cpu_seconds)
@@ -215,6 +217,7 @@ def main():
         "gflops": round(step_flops / (ms_per_step * 1e-3) / 1e9, 1),
         "wall_s_timed": round(wall, 4),
         "roofline": roofline,
+        "achievable_bandwidth": probe,
         "cpu_baseline": cpu,
         "shapes": shapes,
     }
@@ -234,6 +237,32 @@ def main():
         bench_dense(hml)
     if args.grouped:
         bench_grouped(hml)
+
+
+def bandwidth_probe(hml, src, S, nbytes=256 << 20, reps=5):
+    """Achievable HBM bandwidth beside the vendor peak: a device-to-device
+    copy of 256 MiB (hipMemcpy's copy kernel: reads and writes nbytes) and
+    the library's fill kernel over 256 MiB (writes only), HIP-event timed."""
+    dst = hml.DeviceBuffer(nbytes)
+    hml.check(hml.lib.hmlMemcpyDtoD(dst.ptr, src.ptr, nbytes))
+    hml.synchronize()
+    e0, e1 = hml.Event(), hml.Event()
+    e0.record(None)
+    for _ in range(reps):
+        hml.check(hml.lib.hmlMemcpyDtoD(dst.ptr, src.ptr, nbytes))
+    e1.record(None)
+    e1.synchronize()
+    copy = 2.0 * nbytes * reps / (e0.elapsed_ms(e1) * 1e-3) / 1e9
+    e0.record(S)
+    for i in range(reps):
+        dst.fill_uniform(nbytes // 4, 7 + i, stream=S)
+    e1.record(S)
+    e1.synchronize()
+    write = 1.0 * nbytes * reps / (e0.elapsed_ms(e1) * 1e-3) / 1e9
+    dst.free()
+    return {"copy_read_plus_write_GB_s": round(copy, 1), "fill_write_GB_s": round(write, 1),
+            "bytes": nbytes, "frac_of_peak": {"copy": round(copy / HBM_PEAK_GBS, 4),
+                                             "fill": round(write / HBM_PEAK_GBS, 4)}}
 
 
 def bench_dense(hml, reps=5):

@@ -24,10 +24,10 @@ struct DensePlan {
 
 DensePlan plan_dense(int prec, int64_t M, int64_t N, int64_t K) {
   DensePlan d;
-  d.Mp = round_up(std::max<int64_t>(M, 1), kDenseTileM);
-  d.Np = round_up(std::max<int64_t>(N, 1), kDenseTileN);
-  d.Kp = round_up(std::max<int64_t>(K, 1), kDenseTileK);
   const int S = split_terms(prec);
+  d.Mp = round_up(std::max<int64_t>(M, 1), S ? kSplitTileM : kDenseTileM);
+  d.Np = round_up(std::max<int64_t>(N, 1), S ? kSplitTileN : kDenseTileN);
+  d.Kp = round_up(std::max<int64_t>(K, 1), kDenseTileK);
   const size_t elem = S ? 2 : 4;
   const int planes = S ? S : 1;
   d.a_bytes = align256((size_t)planes * d.Mp * d.Kp * elem);

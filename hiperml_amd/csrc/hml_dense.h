@@ -7,13 +7,18 @@
 
 namespace hml {
 
-constexpr int kDenseTileM = 128;
+constexpr int kDenseTileM = 128;  // FP32 kernel block tile
 constexpr int kDenseTileN = 128;
-constexpr int kDenseTileK = 32;  // padding granularity of the split planes
+constexpr int kDenseTileK = 32;   // padding granularity of K
+constexpr int kSplitTileM = 256;  // bf16 kernels' block tile
+constexpr int kSplitTileN = 128;
+constexpr int kSplitKBlock = 32;  // k-block of the bf16 plane layout
 
 struct DenseParams {
-  const void *Ap;  // [S][Mp][Kp] bf16 planes (S > 0) or [Mp][Kp] FP32 (S == 0)
-  const void *Bp;  // [S][Np][Kp] likewise, B transposed (K contiguous)
+  // S == 0: FP32 [Mp][Kp] / [Np][Kp] (B transposed, K contiguous);
+  // S > 0:  bf16 planes, k-blocked [S][Kp/32][Mp][32] / [S][Kp/32][Np][32]
+  const void *Ap;
+  const void *Bp;
   float *C;
   int64_t sCm, sCn;
   int64_t M, N, Mp, Np, Kp;

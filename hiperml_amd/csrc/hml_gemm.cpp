@@ -216,10 +216,15 @@ void heuristic_sgemm(const GemmKey &k, hmlSgemmVariant *v) {
     v->wpb = 4;
     tiles = ceil_div(k.M, 64) * ceil_div(nct, v->ct);
   }
-  // About one resident round of waves (3 per SIMD of 256 CUs for the DOT
-  // kernel: 768 workgroups of 4 waves; the whole split then runs at once),
-  // each K chunk at least 256 long.
-  const int64_t target = k.form == kFormDot ? 3072 : 4096;
+  // About one resident round of waves: for the DOT kernel the number its
+  // register and LDS budget lets the device hold at once (3072 for 2 x 1
+  // tiles on MI355X: 768 workgroups of 4 waves), so the whole split runs
+  // concurrently; each K chunk at least 256 long.
+  int64_t target = 4096;
+  if (k.form == kFormDot) {
+    const int rw = skinny_dot_resident_waves(v->rt, v->ct);
+    target = rw > 0 ? rw : 3072;
+  }
   int64_t ks = tiles >= target ? 1 : (target + tiles / 2) / tiles;
   ks = std::min<int64_t>(ks, std::max<int64_t>(1, k.K / 256));
   ks = std::min<int64_t>(ks, 64);
@@ -322,10 +327,7 @@ hmlStatus run_sgemm(const GemmCanon &g, const GemmKey &k,
     p.kchunk = pl.kchunk;
     p.kgroups = pl.ksplit;
     p.alpha = alpha; p.beta = beta; p.read_c = read_c;
-    const bool avec = ((uintptr_t)g.A % 16 == 0) && (g.sAm % 4 == 0 || g.M == 1);
-    const bool bvec = g.sBk == 1 && ((uintptr_t)g.B % 16 == 0) &&
-                      (g.sBn % 4 == 0 || g.N == 1);
-    hipError_t e = launch_skinny_dot(v.rt, v.ct, avec, bvec, pl.kw, p, st);
+    hipError_t e = launch_skinny_dot(v.rt, v.ct, g.sBk == 1, pl.kw, p, st);
     if (e != hipSuccess) return from_hip(e, "skinny kernel launch");
     if (pl.ksplit > 1) {
       e = launch_reduce(p.P, pl.ksplit, g.M * g.N, g.N, g.C, g.sCm, g.sCn, alpha,
@@ -433,10 +435,9 @@ void sgemm_compat(float *C, const float *A, const float *B, int M, int N,
   }
   const size_t szA = (size_t)M * K, szB = (size_t)K * N, szC = (size_t)M * N;
   float *dA = nullptr, *dB = nullptr, *dC = nullptr;
-  hmlStatus s;
-  if ((s = hmlDeviceMalloc((void **)&dA, szA * 4)) != HML_STATUS_SUCCESS) compat_die("alloc A", s);
-  if ((s = hmlDeviceMalloc((void **)&dB, szB * 4)) != HML_STATUS_SUCCESS) compat_die("alloc B", s);
-  if ((s = hmlDeviceMalloc((void **)&dC, szC * 4)) != HML_STATUS_SUCCESS) compat_die("alloc C", s);
+  std::lock_guard<std::mutex> lk(compat_mutex());
+  hmlStatus s = compat_buffers(szA * 4, szB * 4, szC * 4, &dA, &dB, &dC);
+  if (s != HML_STATUS_SUCCESS) compat_die("alloc A, B, C", s);
   if ((s = hmlMemcpyHtoD(dA, A, szA * 4)) != HML_STATUS_SUCCESS) compat_die("copy A", s);
   if ((s = hmlMemcpyHtoD(dB, B, szB * 4)) != HML_STATUS_SUCCESS) compat_die("copy B", s);
   if ((s = hmlMemcpyHtoD(dC, C, szC * 4)) != HML_STATUS_SUCCESS) compat_die("copy C", s);
@@ -465,9 +466,6 @@ void sgemm_compat(float *C, const float *A, const float *B, int M, int N,
     fprintf(stderr, "Copy C off of device: %s\n", hmlLastError());
     exit(1);
   }
-  hmlDeviceFree(dA);
-  hmlDeviceFree(dB);
-  hmlDeviceFree(dC);
 }
 
 }  // namespace

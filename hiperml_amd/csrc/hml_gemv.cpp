@@ -200,10 +200,9 @@ void sgemv_compat(float *y, const float *A, const float *x, int M, int N,
   }
   const size_t szA = (size_t)M * N;
   float *dA = nullptr, *dx = nullptr, *dy = nullptr;
-  hmlStatus s;
-  if ((s = hmlDeviceMalloc((void **)&dA, szA * 4)) != HML_STATUS_SUCCESS) compat_die("alloc A", s);
-  if ((s = hmlDeviceMalloc((void **)&dx, (size_t)N * 4)) != HML_STATUS_SUCCESS) compat_die("alloc x", s);
-  if ((s = hmlDeviceMalloc((void **)&dy, (size_t)M * 4)) != HML_STATUS_SUCCESS) compat_die("alloc y", s);
+  std::lock_guard<std::mutex> lk(compat_mutex());
+  hmlStatus s = compat_buffers(szA * 4, (size_t)N * 4, (size_t)M * 4, &dA, &dx, &dy);
+  if (s != HML_STATUS_SUCCESS) compat_die("alloc A, x, y", s);
   if ((s = hmlMemcpyHtoD(dA, A, szA * 4)) != HML_STATUS_SUCCESS) compat_die("copy A", s);
   if ((s = hmlMemcpyHtoD(dx, x, (size_t)N * 4)) != HML_STATUS_SUCCESS) compat_die("copy x", s);
   if ((s = hmlMemcpyHtoD(dy, y, (size_t)M * 4)) != HML_STATUS_SUCCESS) compat_die("copy y", s);
@@ -225,9 +224,6 @@ void sgemv_compat(float *y, const float *A, const float *x, int M, int N,
     fprintf(stderr, "Copy y off of device: %s\n", hmlLastError());
     exit(1);
   }
-  hmlDeviceFree(dA);
-  hmlDeviceFree(dx);
-  hmlDeviceFree(dy);
 }
 
 }  // namespace

@@ -1,7 +1,7 @@
 // hml_grouped.cpp -- grouped (batched) skinny SGEMM: a plan holds the device
-// descriptors of many independent row-major problems and a longest-first
-// tile schedule; running it is one launch of the persistent grouped kernel,
-// optionally replayed from a captured hipGraph.
+// descriptors of many independent row-major problems and, per tile class
+// (by N), a longest-first tile schedule; running it launches one persistent
+// kernel per non-empty class, optionally replayed from a captured hipGraph.
 // No reference counterpart (the reference runs one problem per call,
 // src/hml_sgemm.cu:120-211).
 #include <stdio.h>
@@ -18,7 +18,9 @@
 struct hmlGroupedPlan_t {
   int32_t count = 0;
   int64_t ntiles = 0;
-  int nblocks = 0;
+  int64_t cls_off[hml::kGroupClasses] = {};   // tile list of each class
+  int64_t cls_n[hml::kGroupClasses] = {};
+  int cls_blocks[hml::kGroupClasses] = {};
   void *dev = nullptr;  // one allocation: problems | tiles
   hml::GroupProblem *probs = nullptr;
   hml::GroupTile *tiles = nullptr;
@@ -34,8 +36,13 @@ namespace {
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 hmlStatus record(hmlGroupedPlan_t *p, hipStream_t st) {
-  return from_hip(launch_grouped(p->probs, p->tiles, p->ntiles, p->nblocks, st),
-                  "grouped kernel");
+  for (int c = 0; c < kGroupClasses; ++c) {
+    if (!p->cls_n[c]) continue;
+    hipError_t e = launch_grouped(c, p->probs, p->tiles + p->cls_off[c], p->cls_n[c],
+                                  p->cls_blocks[c], st);
+    if (e != hipSuccess) return from_hip(e, "grouped kernel");
+  }
+  return HML_STATUS_SUCCESS;
 }
 
 }  // namespace
@@ -56,73 +63,61 @@ hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan_t **plan, int32_t count,
     return fail(HML_STATUS_INVALID_VALUE, "grouped: NULL argument or count <= 0");
   if (count > 65535) return fail(HML_STATUS_INVALID_VALUE, "grouped: at most 65535 problems");
   std::vector<GroupProblem> probs(count);
-  std::vector<int64_t> tile_cost;
-  std::vector<GroupTile> tiles;
+  std::vector<int64_t> tile_cost[kGroupClasses];
+  std::vector<GroupTile> tiles[kGroupClasses];
   for (int32_t i = 0; i < count; ++i) {
     if (M[i] <= 0 || N[i] <= 0 || K[i] <= 0)
       return fail(HML_STATUS_INVALID_VALUE, "grouped: problem %d has a zero extent", i);
     if (lda[i] < K[i] || ldb[i] < N[i] || ldc[i] < N[i])
       return fail(HML_STATUS_INVALID_VALUE, "grouped: problem %d leading dimension too small", i);
     if (!A[i] || !B[i] || !C[i]) return fail(HML_STATUS_INVALID_VALUE, "grouped: problem %d NULL", i);
-    // Column tiles per pass from the selector (config store / heuristic),
-    // limited to the grouped kernel's instantiations.
-    GemmCanon g = canonicalize(HML_ROW_MAJOR, HML_OP_N, HML_OP_N, M[i], N[i], K[i],
-                               A[i], lda[i], B[i], ldb[i], C[i], ldc[i]);
-    if (g.swapped || g.form != kFormDot)
+    if (N[i] > M[i])
       return fail(HML_STATUS_UNSUPPORTED,
                   "grouped: problem %d has N > M (only tall problems are grouped)", i);
+    const int cls = N[i] <= 16 ? 0 : (N[i] <= 32 ? 1 : 2);
+    int rt, ct;
+    grouped_class_shape(cls, &rt, &ct);
     // the tile body addresses A and B with 31-bit buffer offsets
-    if (((15.0 * lda[i] + K[i] + 4) * 4.0 >= 2147483647.0) ||
-        ((K[i] + 16.0) * ldb[i] + 63.0) * 4.0 >= 2147483647.0)
+    if (((16.0 * rt - 1) * lda[i] + K[i] + 4) * 4.0 >= 2147483647.0 ||
+        ((K[i] + 16.0) * ldb[i] + 16.0 * ct) * 4.0 >= 2147483647.0)
       return fail(HML_STATUS_UNSUPPORTED, "grouped: problem %d strides too large", i);
-    GemmKey key = make_key(g, beta, 0, HML_ROW_MAJOR, HML_OP_N, HML_OP_N);
-    key.gemv_ok = false;
-    hmlSgemmVariant v;
-    int src;
-    select_sgemm(key, &v, &src);
-    const int64_t nct = ceil_div(N[i], 16);
-    // (the grouped kernel instantiates 1 and 2 column tiles per pass: its
-    // register budget then allows 2 waves per SIMD)
-    int ct = (v.family == HML_SGEMM_FAMILY_DOT) ? v.ct : 2;
-    while (ct > 1 && ct / 2 >= nct) ct /= 2;
-    if (ct > 2) ct = 2;
     GroupProblem &p = probs[i];
     p.A = A[i]; p.lda = lda[i]; p.B = B[i]; p.ldb = ldb[i]; p.C = C[i]; p.ldc = ldc[i];
     p.M = M[i]; p.N = N[i]; p.K = K[i];
-    p.ct = ct;
-    p.avec = ((uintptr_t)A[i] % 16 == 0 && lda[i] % 4 == 0) ? 1 : 0;
     p.read_c = beta != 0.0f ? 1 : 0;
     p.alpha = alpha; p.beta = beta;
-    const int64_t nrb = ceil_div(M[i], kDotRowsPerTile);
-    // cost of one 16-row tile: A bytes plus C bytes, in 4-byte words
-    const int64_t cost = 16 * K[i] + 16 * N[i] * (1 + p.read_c) + 64;
+    const int64_t rows = 16 * rt;
+    const int64_t nrb = ceil_div(M[i], rows);
+    const int64_t passes = ceil_div(ceil_div(N[i], 16), ct);
+    // cost of one tile: A bytes per column pass plus C bytes, in 4-byte words
+    const int64_t cost = passes * rows * K[i] + rows * N[i] * (1 + p.read_c) + 64;
     for (int64_t rb = 0; rb < nrb; ++rb) {
-      tiles.push_back(GroupTile{i, 0, rb});
-      tile_cost.push_back(cost);
+      tiles[cls].push_back(GroupTile{i, 0, rb});
+      tile_cost[cls].push_back(cost);
     }
   }
-  // longest-first order (stable: equal costs keep problem order)
-  std::vector<int64_t> order(tiles.size());
-  for (size_t t = 0; t < order.size(); ++t) order[t] = (int64_t)t;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int64_t a, int64_t b) { return tile_cost[a] > tile_cost[b]; });
-  std::vector<GroupTile> sorted(tiles.size());
-  for (size_t t = 0; t < order.size(); ++t) sorted[t] = tiles[order[t]];
-
   hmlGroupedPlan_t *pl = new hmlGroupedPlan_t();
   pl->count = count;
-  pl->ntiles = (int64_t)sorted.size();
-  int dev = 0, cus = 256;
-  if (hipGetDevice(&dev) == hipSuccess) {
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+  std::vector<GroupTile> sorted;
+  for (int c = 0; c < kGroupClasses; ++c) {
+    // longest-first order (stable: equal costs keep problem order)
+    std::vector<int64_t> order(tiles[c].size());
+    for (size_t t = 0; t < order.size(); ++t) order[t] = (int64_t)t;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return tile_cost[c][a] > tile_cost[c][b]; });
+    pl->cls_off[c] = (int64_t)sorted.size();
+    pl->cls_n[c] = (int64_t)order.size();
+    for (int64_t t : order) sorted.push_back(tiles[c][t]);
+    // persistent grid: the workgroups the device holds at once (4 waves
+    // each), never more than needed to give every wave a tile
+    int resident = grouped_class_blocks(c);
+    if (resident <= 0) resident = 512;
+    pl->cls_blocks[c] = (int)std::max<int64_t>(1, std::min<int64_t>(resident, ceil_div(pl->cls_n[c], 4)));
   }
-  // persistent grid: 2 blocks of 4 waves per CU, never more blocks than
-  // needed to give every wave a tile
-  pl->nblocks = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)cus * 2, ceil_div(pl->ntiles, 4)));
+  pl->ntiles = (int64_t)sorted.size();
   const size_t o_probs = 0;
   const size_t o_tiles = align256(o_probs + sizeof(GroupProblem) * count);
-  pl->bytes = align256(o_tiles + sizeof(GroupTile) * sorted.size());
+  pl->bytes = align256(o_tiles + sizeof(GroupTile) * std::max<size_t>(1, sorted.size()));
   hipError_t e = hipMalloc(&pl->dev, pl->bytes);
   if (e != hipSuccess) {
     const size_t b = pl->bytes;
@@ -134,7 +129,7 @@ hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan_t **plan, int32_t count,
   pl->probs = (GroupProblem *)(base + o_probs);
   pl->tiles = (GroupTile *)(base + o_tiles);
   e = hipMemcpy(pl->probs, probs.data(), sizeof(GroupProblem) * count, hipMemcpyHostToDevice);
-  if (e == hipSuccess)
+  if (e == hipSuccess && !sorted.empty())
     e = hipMemcpy(pl->tiles, sorted.data(), sizeof(GroupTile) * sorted.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     (void)hipFree(pl->dev);
@@ -179,7 +174,10 @@ hmlStatus hmlSgemmGroupedInfo(hmlGroupedPlan_t *plan, int64_t *ntiles,
                               int32_t *nblocks, size_t *device_bytes) {
   if (!plan) return fail(HML_STATUS_INVALID_VALUE, "plan is NULL");
   if (ntiles) *ntiles = plan->ntiles;
-  if (nblocks) *nblocks = plan->nblocks;
+  if (nblocks) {
+    *nblocks = 0;
+    for (int c = 0; c < kGroupClasses; ++c) *nblocks += plan->cls_n[c] ? plan->cls_blocks[c] : 0;
+  }
   if (device_bytes) *device_bytes = plan->bytes;
   return HML_STATUS_SUCCESS;
 }

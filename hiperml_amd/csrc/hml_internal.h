@@ -81,9 +81,13 @@ struct DotParams {
 };
 // kw = waves per workgroup (each with its own K chunk)
 bool skinny_dot_runnable(int rt, int ct, int kw);
+// Waves of the (rt, ct) tile kernel with 4-wave workgroups resident on the
+// current device at once (0 if unknown).
+int skinny_dot_resident_waves(int rt, int ct);
 size_t skinny_dot_lds_bytes(int rt, int ct, int kw);
-hipError_t launch_skinny_dot(int rt, int ct, bool avec, bool bvec, int kw,
-                             const DotParams &p, hipStream_t st);
+// bvec: B is K-contiguous (sBk == 1), read with 16-byte loads
+hipError_t launch_skinny_dot(int rt, int ct, bool bvec, int kw, const DotParams &p,
+                             hipStream_t st);
 
 // out(m,n) = alpha * sum_r P[r*total + m*ninner + n] + beta * out(m,n)
 hipError_t launch_reduce(const float *P, int R, int64_t total, int64_t ninner,
@@ -139,16 +143,20 @@ struct GroupProblem {
   float *C;
   int64_t ldc;
   int64_t M, N, K;
-  int32_t ct;       // column tiles per pass: 1 or 2
-  int32_t avec;     // A 16-byte aligned with lda % 4 == 0
   int32_t read_c;
   float alpha, beta;
 };
 struct GroupTile {
   int32_t prob;
   int32_t pad;
-  int64_t rb;  // row block of 16 rows
+  int64_t rb;  // row block of 16 * rt rows (rt of the tile class)
 };
-hipError_t launch_grouped(const GroupProblem *probs, const GroupTile *tiles,
+// Tile classes of the grouped kernel, by N: 0: N <= 16 (2 x 1 tiles of 16 x
+// 16), 1: N <= 32 (2 x 2), 2: larger N (1 x 4, several column passes).
+constexpr int kGroupClasses = 3;
+void grouped_class_shape(int cls, int *rt, int *ct);
+// Workgroups of the class kernel the device holds at once (0 if unknown).
+int grouped_class_blocks(int cls);
+hipError_t launch_grouped(int cls, const GroupProblem *probs, const GroupTile *tiles,
                           int64_t ntiles, int nblocks, hipStream_t st);
 }  // namespace hml

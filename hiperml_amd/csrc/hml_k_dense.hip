@@ -13,12 +13,17 @@
 // are exact in FP32, so the error comes from the dropped terms
 // (2^-16 relative for S = 2, ~2^-24 for S = 3) plus FP32 accumulation.
 //
-// Data path: a split pass writes zero-padded, K-contiguous planes
-// Ap[S][Mp][Kp] and Bp[S][Np][Kp] (B transposed), so the GEMM kernel reads
-// 16-byte chunks with no bounds checks.  Block tile 128 x 128 x 32, 4 waves
-// in a 2 x 2 grid of 64 x 64 wave tiles, two LDS buffers (one barrier per
-// k-step), global loads of step t+1 in flight while step t computes, and an
-// XCD-aware block order (blocks that share A/B panels run on the same L2).
+// Data path: a split pass writes zero-padded bf16 planes in a k-blocked
+// layout, Ap[S][Kp/32][Mp][32] and Bp[S][Kp/32][Np][32] (B transposed): the
+// 32-wide k-step of a 256-row panel is then 16 KB of contiguous memory, read
+// with whole-line 16-byte loads and no bounds checks.  Block tile 256 x 128 x
+// 32, 8 waves in a 4 x 2 grid of 64 x 64 wave tiles (2 waves per SIMD), two
+// LDS buffers (one barrier per k-step), global loads of step t+1 in flight
+// while step t computes, LDS rows unpadded with the 16-byte chunk index
+// XOR-swizzled by the row (conflict-free row-contiguous writes and operand
+// reads), and an XCD-aware block order (blocks that share A/B panels run on
+// the same L2).  The plain FP32 kernel keeps K-contiguous FP32 copies and
+// 128 x 128 tiles.
 #include "hml_internal.h"
 #include "hml_dense.h"
 
@@ -29,10 +34,9 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 
-constexpr int BM = kDenseTileM, BN = kDenseTileN;
-constexpr int BKH = 32;        // k per step, bf16 kernel
+constexpr int BM = kDenseTileM, BN = kDenseTileN;   // FP32 kernel tile
+constexpr int BMH = kSplitTileM, BNH = kSplitTileN;  // bf16 kernels' tile
 constexpr int BKF = 16;        // k per step, FP32 kernel
-constexpr int PADH = 8;        // bf16 elements of LDS row padding
 constexpr int PADF = 4;        // float elements of LDS row padding
 
 // Round-to-nearest-even FP32 -> bf16 bits (finite inputs; NaN kept quiet).
@@ -49,8 +53,12 @@ __device__ __forceinline__ float bf16_to_float(uint32_t b) {
 // ---- split / pack pass -------------------------------------------------------
 // dst plane s (bf16, or FP32 when S == 0) of a rows x K operand X(r,k) =
 // src[r*sR + k*sK], padded to rowsP x Kp.  One thread per 8 consecutive k of
-// one row; the thread index runs fastest along whichever of r / k is
-// contiguous in src so the reads coalesce.
+// one row.  FP32 copies: the thread index runs fastest along whichever of
+// r / k is contiguous in src, so the reads coalesce.  bf16 planes
+// (k-blocked): thread idx writes the 8 elements at idx * 8 of each plane
+// (chunk q = idx % 4 of row (idx / 4) % rowsP in k-block (idx / 4) / rowsP),
+// so the writes are contiguous and the reads cover 128 bytes of a row (K
+// contiguous) or 4 interleaved runs along the rows (rows contiguous).
 template <int S>
 __global__ void k_split(const float *__restrict__ src, int64_t sR, int64_t sK,
                         int64_t rows, int64_t K, int64_t rowsP, int64_t Kp,
@@ -59,17 +67,26 @@ __global__ void k_split(const float *__restrict__ src, int64_t sR, int64_t sK,
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= rowsP * kc_n) return;
   int64_t r, kc;
-  if (kfast) { r = idx / kc_n; kc = idx - r * kc_n; }
-  else { kc = idx / rowsP; r = idx - kc * rowsP; }
+  if (S > 0) {
+    const int64_t rq = idx >> 2;
+    const int64_t kt = rq / rowsP;
+    r = rq - kt * rowsP;
+    kc = kt * 4 + (idx & 3);
+  } else if (kfast) {
+    r = idx / kc_n;
+    kc = idx - r * kc_n;
+  } else {
+    kc = idx / rowsP;
+    r = idx - kc * rowsP;
+  }
   float v[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     const int64_t k = kc * 8 + j;
     v[j] = (r < rows && k < K) ? src[r * sR + k * sK] : 0.f;
   }
-  const int64_t off = r * Kp + kc * 8;
   if (S == 0) {
-    float *d = (float *)dst + off;
+    float *d = (float *)dst + r * Kp + kc * 8;
     *reinterpret_cast<f4 *>(d) = f4{v[0], v[1], v[2], v[3]};
     *reinterpret_cast<f4 *>(d + 4) = f4{v[4], v[5], v[6], v[7]};
     return;
@@ -85,7 +102,7 @@ __global__ void k_split(const float *__restrict__ src, int64_t sR, int64_t sK,
     }
     u4 w = {b[0] | (b[1] << 16), b[2] | (b[3] << 16), b[4] | (b[5] << 16),
             b[6] | (b[7] << 16)};
-    *reinterpret_cast<u4 *>((uint16_t *)dst + s * plane + off) = w;
+    *reinterpret_cast<u4 *>((uint16_t *)dst + s * plane + idx * 8) = w;
   }
 }
 
@@ -118,42 +135,51 @@ __device__ __forceinline__ void store_c(const DenseParams &p, int64_t row,
 }
 
 // ---- split-precision bf16 GEMM ---------------------------------------------
-template <int S, int D>
-__global__ __launch_bounds__(256, (S == 2 ? 2 : 1)) void k_dense_bf16(DenseParams p) {
-  __shared__ __attribute__((aligned(16))) uint16_t As[2][S][BM][BKH + PADH];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][S][BN][BKH + PADH];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
-  int64_t bm, bn;
-  block_coords(p.Mp / BM, p.Np / BN, &bm, &bn);
-  const uint16_t *Ag = (const uint16_t *)p.Ap + bm * BM * p.Kp;
-  const uint16_t *Bg = (const uint16_t *)p.Bp + bn * BN * p.Kp;
-  const int64_t planeA = p.Mp * p.Kp, planeB = p.Np * p.Kp;
+// bf16 element offset of 16-byte chunk kq (0..3) of LDS tile row x (32 bf16
+// per row, unpadded): 16 consecutive rows reading one chunk then cover all
+// 64 banks, and 4 threads writing one row's 4 chunks stay contiguous.
+__device__ __forceinline__ int sw_off(int x, int kq) {
+  return x * kSplitKBlock + 8 * (kq ^ ((x >> 2) & 3));
+}
 
-  // Each thread moves 2 chunks of 8 bf16 per plane per operand per step.
-  // Global loads run D steps ahead in D register sets: the set of step
-  // t + D is loaded while step t computes and stored to LDS at the end of
-  // step t + D - 1, so each load has D compute phases to arrive.
-  u4 ra[D][S][2], rb[D][S][2];
-  auto gload = [&](int set, int64_t k0) {
+template <int S>
+__global__ __launch_bounds__(512) void k_dense_bf16(DenseParams p) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][S][BMH * kSplitKBlock];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][S][BNH * kSplitKBlock];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wr = wave >> 1, wc = wave & 1;  // 4 x 2 waves of 64 x 64
+  int64_t bm, bn;
+  block_coords(p.Mp / BMH, p.Np / BNH, &bm, &bn);
+  const int64_t nsteps = p.Kp / kSplitKBlock;
+  const uint16_t *Ag = (const uint16_t *)p.Ap + bm * BMH * kSplitKBlock;
+  const uint16_t *Bg = (const uint16_t *)p.Bp + bn * BNH * kSplitKBlock;
+  const int64_t planeA = p.Mp * p.Kp, planeB = p.Np * p.Kp;
+  const int64_t slabA = p.Mp * kSplitKBlock, slabB = p.Np * kSplitKBlock;
+
+  // Step t of the block: A rows bm*256 .. +255 (thread chunks c = tid and
+  // tid + 512: row c / 4, chunk c % 4) and B rows bn*128 .. +127 (c = tid),
+  // each a contiguous run of the k-blocked planes.
+  u4 ra[S][2], rb[S];
+  auto gload = [&](int64_t t) {
 #pragma unroll
-    for (int s = 0; s < S; ++s)
+    for (int s = 0; s < S; ++s) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const int c = tid + 256 * i, row = c >> 2, kq = c & 3;
-        ra[set][s][i] = *reinterpret_cast<const u4 *>(Ag + s * planeA + row * p.Kp + k0 + 8 * kq);
-        rb[set][s][i] = *reinterpret_cast<const u4 *>(Bg + s * planeB + row * p.Kp + k0 + 8 * kq);
-      }
+      for (int i = 0; i < 2; ++i)
+        ra[s][i] = *reinterpret_cast<const u4 *>(Ag + s * planeA + t * slabA + (tid + 512 * i) * 8);
+      rb[s] = *reinterpret_cast<const u4 *>(Bg + s * planeB + t * slabB + tid * 8);
+    }
   };
-  auto lstore = [&](int set, int buf) {
+  auto lstore = [&](int buf) {
 #pragma unroll
-    for (int s = 0; s < S; ++s)
+    for (int s = 0; s < S; ++s) {
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        const int c = tid + 256 * i, row = c >> 2, kq = c & 3;
-        *reinterpret_cast<u4 *>(&As[buf][s][row][8 * kq]) = ra[set][s][i];
-        *reinterpret_cast<u4 *>(&Bs[buf][s][row][8 * kq]) = rb[set][s][i];
+        const int c = tid + 512 * i;
+        *reinterpret_cast<u4 *>(&As[buf][s][sw_off(c >> 2, c & 3)]) = ra[s][i];
       }
+      *reinterpret_cast<u4 *>(&Bs[buf][s][sw_off(tid >> 2, tid & 3)]) = rb[s];
+    }
   };
 
   f4 acc[4][4];
@@ -162,20 +188,36 @@ __global__ __launch_bounds__(256, (S == 2 ? 2 : 1)) void k_dense_bf16(DenseParam
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
+  // Step t: read the fragments of step t from LDS buffer t % 2, store step
+  // t + 1 (in registers since step t - 1) into buffer (t + 1) % 2 -- read
+  // by no wave since the barrier that ended step t - 1 -- and reload the
+  // registers with step t + 2; the LDS writes and the global loads then
+  // run under the MFMAs of step t.  One barrier per step.
   const int r16 = lane & 15, g = lane >> 4;
-  const int64_t nsteps = p.Kp / BKH;
-  // step t: LDS buffer t % 2, register set of step t + D: (t + D) % D (both
-  // compile-time in the body below, which the loop runs for even and odd t)
-  auto step = [&](int64_t t, int cur) {
-    if (t + D < nsteps) gload(D == 1 ? 0 : cur, (t + D) * BKH);
+  gload(0);
+  lstore(0);
+  if (nsteps > 1) gload(1);
+  __syncthreads();
+  // planes in the order the products below first need them
+  constexpr int RA3[3] = {1, 2, 0}, RB3[3] = {1, 0, 2};
+  constexpr int RA2[2] = {1, 0}, RB2[2] = {0, 1};
+  for (int64_t t = 0; t < nsteps; ++t) {
+    const int buf = (int)(t & 1);
     bf8 fa[S][4], fb[S][4];
 #pragma unroll
-    for (int s = 0; s < S; ++s)
+    for (int q = 0; q < S; ++q) {
+      const int sa = S == 3 ? RA3[q] : RA2[q];
+      const int sb = S == 3 ? RB3[q] : RB2[q];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        fa[s][i] = *reinterpret_cast<const bf8 *>(&As[cur][s][wr * 64 + i * 16 + r16][8 * g]);
-        fb[s][i] = *reinterpret_cast<const bf8 *>(&Bs[cur][s][wc * 64 + i * 16 + r16][8 * g]);
+        fa[sa][i] = *reinterpret_cast<const bf8 *>(&As[buf][sa][sw_off(wr * 64 + i * 16 + r16, g)]);
+        fb[sb][i] = *reinterpret_cast<const bf8 *>(&Bs[buf][sb][sw_off(wc * 64 + i * 16 + r16, g)]);
       }
+    }
+    if (t + 1 < nsteps) {
+      lstore(buf ^ 1);
+      if (t + 2 < nsteps) gload(t + 2);
+    }
     // products, smallest terms first: (S==3) m*m, l*h, h*l, m*h, h*m, h*h;
     // (S==2) l*h, h*l, h*h.  The product loop is outermost so that
     // consecutive MFMAs update different tiles (no back-to-back dependency);
@@ -193,19 +235,8 @@ __global__ __launch_bounds__(256, (S == 2 ? 2 : 1)) void k_dense_bf16(DenseParam
         for (int j = 0; j < 4; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ia][i], fb[ib][j], acc[i][j], 0, 0, 0);
     }
-    if (t + 1 < nsteps) lstore(D == 1 ? 0 : cur ^ 1, cur ^ 1);
     __syncthreads();
-  };
-  gload(0, 0);
-  lstore(0, 0);
-  if (D == 2 && nsteps > 1) gload(1, BKH);
-  __syncthreads();
-  int64_t t = 0;
-  for (; t + 1 < nsteps; t += 2) {
-    step(t, 0);
-    step(t + 1, 1);
   }
-  if (t < nsteps) step(t, 0);
 
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -213,8 +244,8 @@ __global__ __launch_bounds__(256, (S == 2 ? 2 : 1)) void k_dense_bf16(DenseParam
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e)
-        store_c(p, bm * BM + wr * 64 + i * 16 + 4 * g + e,
-                bn * BN + wc * 64 + j * 16 + r16, acc[i][j][e]);
+        store_c(p, bm * BMH + wr * 64 + i * 16 + 4 * g + e,
+                bn * BNH + wc * 64 + j * 16 + r16, acc[i][j][e]);
 }
 
 // ---- plain FP32 GEMM on v_mfma_f32_16x16x4f32 ------------------------------
@@ -301,18 +332,15 @@ hipError_t launch_split(const float *src, int64_t sR, int64_t sK, int64_t rows,
   return hipGetLastError();
 }
 
-// global prefetch depth (steps) of the bf16x3 / bf16x6 kernels
-constexpr int kPrefetch2 = 2, kPrefetch3 = 2;
-
 hipError_t launch_dense(int S, const DenseParams &p, hipStream_t st) {
-  const int64_t nblk = (p.Mp / BM) * (p.Np / BN);
+  const int64_t nblk = S == 0 ? (p.Mp / BM) * (p.Np / BN) : (p.Mp / BMH) * (p.Np / BNH);
   if (nblk <= 0) return hipSuccess;
   if (nblk > 0x7fffffff) return hipErrorInvalidConfiguration;
   const dim3 grid((unsigned)nblk);
   switch (S) {
     case 0: hipLaunchKernelGGL(k_dense_f32, grid, dim3(256), 0, st, p); break;
-    case 2: hipLaunchKernelGGL((k_dense_bf16<2, kPrefetch2>), grid, dim3(256), 0, st, p); break;
-    case 3: hipLaunchKernelGGL((k_dense_bf16<3, kPrefetch3>), grid, dim3(256), 0, st, p); break;
+    case 2: hipLaunchKernelGGL(k_dense_bf16<2>, grid, dim3(512), 0, st, p); break;
+    case 3: hipLaunchKernelGGL(k_dense_bf16<3>, grid, dim3(512), 0, st, p); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -21,6 +21,9 @@
 //  * K is split across waves when rows alone cannot fill 256 CUs; the
 //    partial tiles are summed in a fixed order by a second kernel
 //    (deterministic, no atomics).
+#include <algorithm>
+#include <mutex>
+
 #include "hml_internal.h"
 
 namespace hml {
@@ -113,16 +116,13 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base,
                                                            int32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, bytes, 0x00020000);
 }
-template <bool VEC, int AUX>
+// 16-byte load at any 4-byte-aligned offset: buffer_load_dwordx4 needs only
+// dword alignment on MI355X (checked bit-exact with odd leading dimensions
+// by the parity tests).
+template <int AUX>
 __device__ __forceinline__ f4 buf_load4(__amdgpu_buffer_rsrc_t rs, uint32_t voff,
                                         int32_t soff) {
-  if (VEC)
-    return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, AUX));
-  f4 v;
-#pragma unroll
-  for (int j = 0; j < 4; ++j)
-    v[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff + 4 * j, soff, AUX));
-  return v;
+  return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, AUX));
 }
 __device__ __forceinline__ float buf_load1(__amdgpu_buffer_rsrc_t rs, uint32_t voff,
                                            int32_t soff) {
@@ -146,7 +146,7 @@ __device__ __forceinline__ void dot_store(const DotParams &p, int64_t row,
 // One wave: rows [row0, row0 + 16 RT) x columns [ct0*16, ct0*16 + 16 CT)
 // over k in [kb, ke) (kb a multiple of the 64-k stage), accumulated into
 // acc; T is the wave's private LDS tile (16 RT x 64 floats).
-template <int RT, int CT, bool AVEC, bool BVEC>
+template <int RT, int CT, bool BVEC>
 __device__ __forceinline__ void dot_tile_lds(const float *__restrict__ A, int64_t sAm,
                                              const float *__restrict__ B, int64_t sBk,
                                              int64_t sBn, int64_t M, int64_t N,
@@ -190,7 +190,7 @@ __device__ __forceinline__ void dot_tile_lds(const float *__restrict__ A, int64_
     const __amdgpu_buffer_rsrc_t rs = s < nst ? ra : rnull;
     const int32_t so = (int32_t)(s * kDotStageK * 4);
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) a[j][q] = buf_load4<AVEC, kBufNT>(rs, aoff[q], so);
+    for (int q = 0; q < NQ; ++q) a[j][q] = buf_load4<kBufNT>(rs, aoff[q], so);
   };
   auto loadB = [&](int j, int64_t s) {
     const __amdgpu_buffer_rsrc_t rs = s < nst ? rbs : rnull;
@@ -203,7 +203,7 @@ __device__ __forceinline__ void dot_tile_lds(const float *__restrict__ A, int64_
           const int64_t ko = s * kDotStageK + 32 * u + 16 * v;
           f4 x;
           if (BVEC) {
-            x = buf_load4<true, 0>(rs, boff[c], (int32_t)(ko * 4));
+            x = buf_load4<0>(rs, boff[c], (int32_t)(ko * 4));
           } else {
 #pragma unroll
             for (int jj = 0; jj < 4; ++jj)
@@ -303,7 +303,7 @@ __device__ __forceinline__ void dot_tile_lds(const float *__restrict__ A, int64_
 
 }
 
-template <int RT, int CT, bool AVEC, bool BVEC>
+template <int RT, int CT, bool BVEC>
 __global__ __launch_bounds__(64 * dot_max_waves(RT, CT)) void k_skinny_dot(DotParams p) {
   constexpr int WF = dot_wave_floats(RT, CT);
   extern __shared__ f4 dot_lds[];
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(64 * dot_max_waves(RT, CT)) void k_skinny_dot(DotPa
   int64_t ke = kb + p.kchunk;
   if (ke > p.K) ke = p.K;
   f4 acc[RT][CT];
-  dot_tile_lds<RT, CT, AVEC, BVEC>(p.A, p.sAm, p.B, p.sBk, p.sBn, p.M, p.N, row0,
+  dot_tile_lds<RT, CT, BVEC>(p.A, p.sAm, p.B, p.sBk, p.sBn, p.M, p.N, row0,
                                    ct0, kb, ke, reinterpret_cast<float *>(dot_lds) + w * WF,
                                    acc);
 
@@ -358,19 +358,22 @@ __global__ __launch_bounds__(64 * dot_max_waves(RT, CT)) void k_skinny_dot(DotPa
 
 // ---------------------------------------------------------------------------
 // Persistent grouped SGEMM: hundreds of independent skinny problems in one
-// launch.  The host builds one descriptor per problem and a list of
-// (problem, 16-row block) tiles sorted by decreasing cost; wave w of the
-// persistent grid takes tiles w, w + W, w + 2W, ... (W = waves launched), a
-// static longest-first schedule that needs no atomics or counters, so the
-// launch can be captured in a graph and replayed.  Each tile runs the DOT
-// tile body above (B read in place, whole K, no split) over its problem's
-// column tiles, 1 or 2 per pass (more would cost the register budget of a
-// second wave per SIMD), dispatching on that count and A's alignment.
+// graph launch.  The host sorts the problems into tile classes by N (2 x 1,
+// 2 x 2 or 1 x 4 tiles of 16-row x 16-column blocks), builds one descriptor
+// per problem and, per class, a list of (problem, row block) tiles sorted by
+// decreasing cost; one persistent kernel per class runs its list: wave w
+// takes tiles w, w + W, w + 2W, ... (W = waves launched), a static
+// longest-first schedule with no atomics or counters, so the launches can
+// be captured in a graph and replayed.  Each tile runs the DOT tile body
+// above (B read in place, whole K, no split) over its problem's column
+// tiles.  One kernel per class keeps each at its own register budget (a
+// single kernel dispatching on the class would pay for the largest).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_grouped_dot(const GroupProblem *probs,
+template <int RT, int CT>
+__global__ __launch_bounds__(256, 2) void k_grouped_dot(const GroupProblem *probs,
                                                      const GroupTile *tiles,
                                                      int64_t ntiles) {
-  __shared__ f4 gtile[4][dot_wave_floats(1, 1) / 4];
+  __shared__ f4 gtile[4][dot_wave_floats(RT, CT) / 4];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int r = lane & 15, g = lane >> 4;
@@ -380,42 +383,27 @@ __global__ __launch_bounds__(256) void k_grouped_dot(const GroupProblem *probs,
   for (int64_t t = w; t < ntiles; t += W) {
     const GroupTile tile = tiles[t];
     const GroupProblem &gp = probs[tile.prob];
-    const int64_t row0 = tile.rb * 16;
+    const int64_t row0 = tile.rb * 16 * RT;
     const int64_t nct = (gp.N + 15) / 16;
-    for (int ct0 = 0; ct0 < nct; ct0 += gp.ct) {
-      f4 acc[1][2];
-      switch (gp.ct * 2 + gp.avec) {
-#define HML_GROUP_CASE(CT, AV)                                                        \
-  case CT * 2 + AV: {                                                                \
-    f4 a1[1][CT];                                                                    \
-    dot_tile_lds<1, CT, AV, false>(gp.A, gp.lda, gp.B, gp.ldb, 1, gp.M, gp.N, row0,  \
-                                   ct0, 0, gp.K, T, a1);                             \
-    _Pragma("unroll") for (int c = 0; c < CT; ++c) acc[0][c] = a1[0][c];             \
-    break;                                                                           \
-  }
-        HML_GROUP_CASE(1, 0)
-        HML_GROUP_CASE(1, 1)
-        HML_GROUP_CASE(2, 0)
-        HML_GROUP_CASE(2, 1)
-#undef HML_GROUP_CASE
-        default:
-          continue;
-      }
+    for (int ct0 = 0; ct0 < nct; ct0 += CT) {
+      f4 acc[RT][CT];
+      dot_tile_lds<RT, CT, false>(gp.A, gp.lda, gp.B, gp.ldb, 1, gp.M, gp.N, row0,
+                                  ct0, 0, gp.K, T, acc);
 #pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        if (c >= gp.ct) break;
+      for (int tt = 0; tt < RT; ++tt)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int64_t row = row0 + 4 * g + i;
-          const int64_t col = (int64_t)(ct0 + c) * 16 + r;
-          if (row < gp.M && col < gp.N) {
-            float *cp = gp.C + row * gp.ldc + col;
-            float v = gp.alpha * acc[0][c][i];
-            if (gp.read_c) v = gp.alpha * acc[0][c][i] + gp.beta * (*cp);
-            *cp = v;
+        for (int c = 0; c < CT; ++c)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int64_t row = row0 + 16 * tt + 4 * g + i;
+            const int64_t col = (int64_t)(ct0 + c) * 16 + r;
+            if (row < gp.M && col < gp.N) {
+              float *cp = gp.C + row * gp.ldc + col;
+              float v = gp.alpha * acc[tt][c][i];
+              if (gp.read_c) v = gp.alpha * acc[tt][c][i] + gp.beta * (*cp);
+              *cp = v;
+            }
           }
-        }
-      }
     }
   }
 }
@@ -668,7 +656,7 @@ __global__ void k_fill(float *dst, size_t n, float value) {
   for (; i < n; i += stride) dst[i] = value;
 }
 
-template <int RT, int CT, bool AV, bool BV>
+template <int RT, int CT, bool BV>
 hipError_t run_dot(int kw, const DotParams &p, hipStream_t st) {
   const int64_t nrb = (p.M + 16 * RT - 1) / (16 * RT);
   const int64_t nct = (p.N + 15) / 16;
@@ -676,7 +664,7 @@ hipError_t run_dot(int kw, const DotParams &p, hipStream_t st) {
   const int64_t nblk = nrb * p.kgroups;
   if (nblk > 0x7fffffff || ncb > 65535) return hipErrorInvalidConfiguration;
   const size_t lds = (size_t)kw * dot_wave_floats(RT, CT) * sizeof(float);
-  hipLaunchKernelGGL((k_skinny_dot<RT, CT, AV, BV>), dim3((unsigned)nblk, (unsigned)ncb),
+  hipLaunchKernelGGL((k_skinny_dot<RT, CT, BV>), dim3((unsigned)nblk, (unsigned)ncb),
                      dim3(64 * kw), lds, st, p);
   return hipGetLastError();
 }
@@ -684,24 +672,36 @@ hipError_t run_dot(int kw, const DotParams &p, hipStream_t st) {
 typedef hipError_t (*DotFn)(int, const DotParams &, hipStream_t);
 
 template <int RT, int CT>
-DotFn dot_fn(bool av, bool bv) {
-  return av ? (bv ? run_dot<RT, CT, true, true> : run_dot<RT, CT, true, false>)
-            : (bv ? run_dot<RT, CT, false, true> : run_dot<RT, CT, false, false>);
+DotFn dot_fn(bool bv) {
+  return bv ? run_dot<RT, CT, true> : run_dot<RT, CT, false>;
 }
 
 // instantiated (rt, ct): ct <= 4, rt * ct <= 8
-DotFn find_dot(int rt, int ct, bool av, bool bv) {
+DotFn find_dot(int rt, int ct, bool bv) {
   switch (rt * 16 + ct) {
-    case 17: return dot_fn<1, 1>(av, bv);
-    case 18: return dot_fn<1, 2>(av, bv);
-    case 20: return dot_fn<1, 4>(av, bv);
-    case 33: return dot_fn<2, 1>(av, bv);
-    case 34: return dot_fn<2, 2>(av, bv);
-    case 36: return dot_fn<2, 4>(av, bv);
-    case 65: return dot_fn<4, 1>(av, bv);
-    case 66: return dot_fn<4, 2>(av, bv);
+    case 17: return dot_fn<1, 1>(bv);
+    case 18: return dot_fn<1, 2>(bv);
+    case 20: return dot_fn<1, 4>(bv);
+    case 33: return dot_fn<2, 1>(bv);
+    case 34: return dot_fn<2, 2>(bv);
+    case 36: return dot_fn<2, 4>(bv);
+    case 65: return dot_fn<4, 1>(bv);
+    case 66: return dot_fn<4, 2>(bv);
     default: return nullptr;
   }
+}
+
+// Workgroups (256 threads, dynamic LDS `lds`) of kernel `fn` the current
+// device holds at once, from the occupancy calculator; 0 if unknown.
+int resident_blocks(const void *fn, size_t lds) {
+  int nb = 0, dev = 0;
+  hipDeviceProp_t prop;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, lds) != hipSuccess ||
+      hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return std::max(1, nb) * prop.multiProcessorCount;
 }
 
 template <int RT, int CT, bool VEC>
@@ -748,7 +748,7 @@ RunFn find_skinny(int form, int rt, int ct, bool vec) {
 }  // namespace
 
 bool skinny_has_variant(int form, int rt, int ct) {
-  if (form == kFormDot) return find_dot(rt, ct, true, true) != nullptr;
+  if (form == kFormDot) return find_dot(rt, ct, true) != nullptr;
   return find_skinny(form, rt, ct, true) != nullptr;
 }
 
@@ -756,19 +756,42 @@ size_t skinny_dot_lds_bytes(int rt, int ct, int kw) {
   return (size_t)kw * dot_wave_floats(rt, ct) * sizeof(float);
 }
 
+int skinny_dot_resident_waves(int rt, int ct) {
+  // waves of k_skinny_dot<rt, ct> (4-wave workgroups, B read with scalar
+  // loads) that the current device holds at once; cached
+  static std::mutex mu;
+  static int cache[5][5];
+  if (rt < 1 || rt > 4 || ct < 1 || ct > 4 || !find_dot(rt, ct, false)) return 0;
+  std::lock_guard<std::mutex> lk(mu);
+  if (cache[rt][ct]) return cache[rt][ct];
+  const void *fn = nullptr;
+  switch (rt * 16 + ct) {
+    case 17: fn = (const void *)k_skinny_dot<1, 1, false>; break;
+    case 18: fn = (const void *)k_skinny_dot<1, 2, false>; break;
+    case 20: fn = (const void *)k_skinny_dot<1, 4, false>; break;
+    case 33: fn = (const void *)k_skinny_dot<2, 1, false>; break;
+    case 34: fn = (const void *)k_skinny_dot<2, 2, false>; break;
+    case 36: fn = (const void *)k_skinny_dot<2, 4, false>; break;
+    case 65: fn = (const void *)k_skinny_dot<4, 1, false>; break;
+    case 66: fn = (const void *)k_skinny_dot<4, 2, false>; break;
+    default: return 0;
+  }
+  cache[rt][ct] = 4 * resident_blocks(fn, skinny_dot_lds_bytes(rt, ct, 4));
+  return cache[rt][ct];
+}
+
 bool skinny_dot_runnable(int rt, int ct, int kw) {
-  if (!find_dot(rt, ct, true, true)) return false;
+  if (!find_dot(rt, ct, true)) return false;
   return kw >= 1 && kw <= dot_max_waves(rt, ct) && skinny_dot_lds_bytes(rt, ct, kw) <= kDotMaxLds;
 }
 
-hipError_t launch_skinny_dot(int rt, int ct, bool avec, bool bvec, int kw,
-                             const DotParams &p, hipStream_t st) {
+hipError_t launch_skinny_dot(int rt, int ct, bool bvec, int kw, const DotParams &p,
+                             hipStream_t st) {
   if (!skinny_dot_runnable(rt, ct, kw) || p.kgroups < 1 || p.kchunk % kDotStageK != 0)
     return hipErrorInvalidValue;
-  DotFn fn = find_dot(rt, ct, avec, bvec);
+  DotFn fn = find_dot(rt, ct, bvec);
   return fn(kw, p, st);
 }
-
 hipError_t launch_skinny(int form, int rt, int ct, bool vec, int wpb,
                          const SkinnyParams &p, hipStream_t st) {
   RunFn fn = find_skinny(form, rt, ct, vec);
@@ -832,14 +855,42 @@ hipError_t launch_fill(float *dst, size_t n, float value, hipStream_t st) {
 
 namespace hml {
 
-hipError_t launch_grouped(const GroupProblem *probs, const GroupTile *tiles,
+namespace {
+const void *grouped_fn(int cls) {
+  switch (cls) {
+    case 0: return (const void *)k_grouped_dot<2, 1>;
+    case 1: return (const void *)k_grouped_dot<2, 2>;
+    case 2: return (const void *)k_grouped_dot<1, 4>;
+    default: return nullptr;
+  }
+}
+}  // namespace
+
+void grouped_class_shape(int cls, int *rt, int *ct) {
+  static const int kRt[kGroupClasses] = {2, 2, 1}, kCt[kGroupClasses] = {1, 2, 4};
+  *rt = kRt[cls];
+  *ct = kCt[cls];
+}
+
+int grouped_class_blocks(int cls) {
+  static std::mutex mu;
+  static int cache[kGroupClasses];
+  std::lock_guard<std::mutex> lk(mu);
+  if (!cache[cls]) cache[cls] = resident_blocks(grouped_fn(cls), 0);
+  return cache[cls];
+}
+
+hipError_t launch_grouped(int cls, const GroupProblem *probs, const GroupTile *tiles,
                           int64_t ntiles, int nblocks, hipStream_t st) {
   if (ntiles <= 0) return hipSuccess;
   if (nblocks < 1) nblocks = 1;
-  hipLaunchKernelGGL(k_grouped_dot, dim3((unsigned)nblocks), dim3(256), 0, st,
-                     probs, tiles, ntiles);
+  switch (cls) {
+    case 0: hipLaunchKernelGGL((k_grouped_dot<2, 1>), dim3((unsigned)nblocks), dim3(256), 0, st, probs, tiles, ntiles); break;
+    case 1: hipLaunchKernelGGL((k_grouped_dot<2, 2>), dim3((unsigned)nblocks), dim3(256), 0, st, probs, tiles, ntiles); break;
+    case 2: hipLaunchKernelGGL((k_grouped_dot<1, 4>), dim3((unsigned)nblocks), dim3(256), 0, st, probs, tiles, ntiles); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
-
 
 }  // namespace hml

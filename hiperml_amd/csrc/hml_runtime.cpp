@@ -91,6 +91,47 @@ hmlStatus library_workspace(size_t bytes, void **ptr) {
   return HML_STATUS_SUCCESS;
 }
 
+// Device copies of the host operands of the reference-compatible entries:
+// one grow-only allocation per device (A | B | C regions, 256-byte aligned),
+// reused across calls instead of the reference's allocate / free per call.
+namespace {
+std::mutex g_compat_mutex;
+std::vector<DeviceScratch> g_compat;
+}  // namespace
+
+std::mutex &compat_mutex() { return g_compat_mutex; }
+
+hmlStatus compat_buffers(size_t a_bytes, size_t b_bytes, size_t c_bytes,
+                         float **a, float **b, float **c) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return from_hip(e, "hipGetDevice");
+  auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+  const size_t need = al(a_bytes) + al(b_bytes) + al(c_bytes);
+  if ((int)g_compat.size() <= dev) g_compat.resize(dev + 1);
+  DeviceScratch &w = g_compat[dev];
+  if (w.bytes < need) {
+    if (w.ptr) {
+      e = hipFree(w.ptr);
+      w.ptr = nullptr;
+      w.bytes = 0;
+      if (e != hipSuccess) return from_hip(e, "hipFree(staging)");
+    }
+    e = hipMalloc(&w.ptr, need > 0 ? need : 256);
+    if (e != hipSuccess) {
+      w.ptr = nullptr;
+      return fail(HML_STATUS_ALLOC_FAILED, "staging hipMalloc(%zu): %s", need,
+                  hipGetErrorString(e));
+    }
+    w.bytes = need;
+  }
+  char *base = (char *)w.ptr;
+  *a = (float *)base;
+  *b = (float *)(base + al(a_bytes));
+  *c = (float *)(base + al(a_bytes) + al(b_bytes));
+  return HML_STATUS_SUCCESS;
+}
+
 }  // namespace hml
 
 using namespace hml;

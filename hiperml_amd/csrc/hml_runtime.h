@@ -2,6 +2,8 @@
 // error reporting, per-device workspace, argument helpers.
 #pragma once
 
+#include <mutex>
+
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
@@ -31,5 +33,12 @@ int verbosity();
 // Compat-mode failure: print and exit(1), as the reference does
 // (src/hml_sgemm.cu:189-199, src/hml_utils.cu:75-82).
 [[noreturn]] void compat_die(const char *where, hmlStatus s);
+
+// Device copies of the compat entries' host operands: A | B | C regions of
+// one grow-only per-device allocation.  Hold compat_mutex() while using them
+// (one compat call at a time per process).
+std::mutex &compat_mutex();
+hmlStatus compat_buffers(size_t a_bytes, size_t b_bytes, size_t c_bytes,
+                         float **a, float **b, float **c);
 
 }  // namespace hml

@@ -141,13 +141,14 @@ size_t hmlSgemmDenseWorkspaceSize(int32_t precision, int64_t M, int64_t N,
 /* Grouped skinny SGEMM, no reference counterpart: C_i = alpha * A_i * B_i +
  * beta * C_i for `count` independent ROW-MAJOR problems (M_i >= N_i), given
  * as host arrays of device pointers and extents.  The plan owns the device
- * descriptors and a longest-first tile schedule for one persistent kernel
- * (column tiles per problem from the selector).  Run computes every C_i in
- * one launch; with use_graph that launch is captured into a hipGraph on the
- * first run and replayed after.  The A/B/C pointers are captured at
- * creation: rerun after changing their contents, create a new plan for new
- * pointers.  HML_STATUS_UNSUPPORTED for N_i > M_i, or strides so large that
- * 15 * lda_i + K_i or (K_i + 16) * ldb_i reaches 2^29 elements. */
+ * descriptors and, per tile class (N <= 16, N <= 32, larger), a longest-first
+ * tile schedule for one persistent kernel.  Run launches one kernel per
+ * non-empty class; with use_graph those launches are captured into a
+ * hipGraph on the first run and replayed after.  The A/B/C pointers are
+ * captured at creation: rerun after changing their contents, create a new
+ * plan for new pointers.  HML_STATUS_UNSUPPORTED for N_i > M_i, or strides
+ * so large that 31 * lda_i + K_i or (K_i + 16) * ldb_i reaches 2^29
+ * elements. */
 typedef struct hmlGroupedPlan_t *hmlGroupedPlan;
 hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan *plan, int32_t count,
                                 const int64_t *M, const int64_t *N,
