@@ -54,6 +54,8 @@ def main():
                     help="also run the split-precision dense suite (extra JSON line)")
     ap.add_argument("--grouped", action="store_true",
                     help="also run the grouped 512-problem suite (extra JSON line)")
+    ap.add_argument("--kmeans", action="store_true",
+                    help="also run the k-means assignment suite (extra JSON line)")
     ap.add_argument("--only-extra", action="store_true",
                     help="skip the main sweep, run only the requested extra suites")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -82,6 +84,8 @@ def main():
             bench_dense(hml)
         if args.grouped:
             bench_grouped(hml)
+        if args.kmeans:
+            bench_kmeans(hml, cpu=not args.no_cpu)
         return
     stream = hml.Stream()
     S = stream.ptr
@@ -237,6 +241,8 @@ def main():
         bench_dense(hml)
     if args.grouped:
         bench_grouped(hml)
+    if args.kmeans:
+        bench_kmeans(hml, cpu=not args.no_cpu)
 
 
 def bandwidth_probe(hml, src, S, nbytes=256 << 20, reps=5):
@@ -371,6 +377,62 @@ def bench_grouped(hml, count=512, reps=5, seed=0):
                       "grouped_graph_GB_s": round(nbytes / t_graph / 1e6, 1),
                       "tiles": info["tiles"], "blocks": info["blocks"],
                       "max_err_over_bound_first64": worst}))
+
+
+def bench_kmeans(hml, reps=5, rows=1 << 24, cpu=True):
+    """k-means assignment (hmlKmeansAssign) of 2^24 rows, uniform[-1,1]
+    rows and centroids, for dims 16/32/64 x clusters 16/64/256.  Algorithmic
+    bytes per row: 4 dims (row) + 4 (index); flops: 2 dims clusters (the
+    x.c screen on the matrix cores).  The CPU column times the restated
+    reference assign loop (oracle, 1 thread) on the first 20000 rows and
+    counts that sample's mismatches."""
+    import numpy as np
+    S = hml.Stream()
+    out = []
+    for D in (16, 32, 64):
+        R = hml.DeviceBuffer(4 * rows * D)
+        R.fill_uniform(rows * D, 11 + D, stream=S.ptr)
+        A = hml.DeviceBuffer(4 * rows)
+        for K in (16, 64, 256):
+            C = hml.DeviceBuffer(4 * K * D)
+            C.fill_uniform(K * D, 5 + K, stream=S.ptr)
+            hml.kmeans_assign(R, C, A, rows, D, K, stream=S.ptr)
+            S.synchronize()
+            e0, e1 = hml.Event(), hml.Event()
+            e0.record(S.ptr)
+            for _ in range(reps):
+                hml.kmeans_assign(R, C, A, rows, D, K, stream=S.ptr)
+            e1.record(S.ptr)
+            e1.synchronize()
+            ms = e0.elapsed_ms(e1) / reps
+            nbytes = rows * (4.0 * D + 4.0)
+            flops = 2.0 * rows * D * K
+            gbs = nbytes / ms / 1e6
+            tfs = flops / ms / 1e9
+            t_hbm = nbytes / (HBM_PEAK_GBS * 1e9)
+            t_mfma = flops / (FP32_PEAK_TFLOPS * 1e12)
+            rec = {"dims": D, "clusters": K, "ms": round(ms, 4),
+                   "rows_per_s": round(rows / ms * 1e3, 0), "GB_s": round(gbs, 1),
+                   "TFLOP_s": round(tfs, 2),
+                   "bound": "hbm" if t_hbm >= t_mfma else "mfma",
+                   "frac_of_roofline": round(max(t_hbm, t_mfma) / (ms * 1e-3), 4)}
+            if cpu:
+                sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle"))
+                import oracle
+                n = 20000
+                xs = R.to_numpy((n, D))
+                cs = C.to_numpy((K, D))
+                t0 = time.perf_counter()
+                want = oracle.kmeans_assign(xs, cs)
+                dt = time.perf_counter() - t0
+                got = A.to_numpy((n,), np.uint32)
+                rec["cpu_rows_per_s_1thread"] = round(n / dt, 0)
+                rec["cpu_sample_mismatches"] = int((got != want).sum())
+            out.append(rec)
+            C.free()
+        R.free()
+        A.free()
+    print(json.dumps({"suite": "k-means assignment, %d rows" % rows, "results": out}))
 
 
 CALIB_BYTES = 256 << 20

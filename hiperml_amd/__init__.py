@@ -207,6 +207,20 @@ def sgemv_ex(M, N, A, x, y, alpha=1.0, beta=0.0, order="row", trans=False,
 PRECISION = {"fp32": 0, "bf16x3": 1, "bf16x6": 2}
 
 
+KMEANS_SEPARATE_ROUNDING = 0x1
+
+
+def kmeans_assign(rows, ctrds, assign, num_rows, num_dims, num_clusts,
+                  separate_rounding=False, stream=None):
+    """assign[r] = nearest centroid of row r (hmlKmeansAssign) on device
+    buffers: rows num_rows x num_dims, ctrds num_clusts x num_dims, both
+    row-major float32; assign uint32."""
+    check(lib.hmlKmeansAssign(_ptr(assign), _ptr(rows), num_rows, num_dims,
+                              _ptr(ctrds), num_clusts,
+                              KMEANS_SEPARATE_ROUNDING if separate_rounding else 0,
+                              stream))
+
+
 def sgemm_dense(M, N, K, A, B, C, alpha=1.0, beta=0.0, precision="fp32",
                 order="row", transA=False, transB=False, lda=None, ldb=None,
                 ldc=None, flags=0, workspace=None, ws_bytes=0, stream=None):

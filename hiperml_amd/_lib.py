@@ -171,6 +171,8 @@ _SIGS = {
                                  ctypes.c_float, c_void_p, c_i64, c_void_p,
                                  c_i64, ctypes.c_float, c_void_p, c_void_p,
                                  c_i64, c_double_p]),
+    "hmlKmeansAssign": (c_i32, [c_void_p, c_void_p, c_i64, c_i32, c_void_p,
+                                c_i32, c_u32, c_void_p]),
     "hmlInit": (c_i32, []),
     "hmlDeviceCount": (ctypes.c_int, []),
     "hmlSetDevice": (c_i32, [ctypes.c_int]),

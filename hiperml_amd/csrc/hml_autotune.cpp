@@ -155,6 +155,14 @@ std::vector<hmlSgemmVariant> sgemm_candidates(const GemmKey &k) {
     hmlSgemmVariant v;
     if (selector_variant(k, c, &v)) push_unique(&out, v);
   }
+  // Shared-B DOT tiles for wide column blocks (N > 32), K split 1..16.
+  if (k.form == kFormDot && k.N > 32) {
+    for (int rt : {1, 2})
+      for (int ks : {1, 2, 4, 8, 16}) {
+        hmlSgemmVariant v{HML_SGEMM_FAMILY_DOTB, rt, 4, ks, 4};
+        if (ks <= std::max<int64_t>(1, k.K / 256) && sgemm_variant_runnable(k, v)) push_unique(&out, v);
+      }
+  }
   if (sgemm_variant_runnable(k, h)) push_unique(&out, h);
   return out;
 }

@@ -140,8 +140,14 @@ SkinnyPlan plan_skinny(const GemmKey &k, const hmlSgemmVariant &v) {
     int64_t kchunk = ceil_div(ceil_div(Kc, ks), kDotStageK) * kDotStageK;
     const int64_t nchunks = ceil_div(Kc, kchunk);
     pl.kchunk = kchunk;
-    pl.kw = (int)std::max<int64_t>(1, std::min<int64_t>(std::max(1, v.wpb), nchunks));
-    pl.ksplit = (int)ceil_div(nchunks, pl.kw);
+    if (v.family == HML_SGEMM_FAMILY_DOTB) {
+      // the waves of a workgroup take rows: every chunk is a workgroup
+      pl.kw = 1;
+      pl.ksplit = (int)nchunks;
+    } else {
+      pl.kw = (int)std::max<int64_t>(1, std::min<int64_t>(std::max(1, v.wpb), nchunks));
+      pl.ksplit = (int)ceil_div(nchunks, pl.kw);
+    }
     pl.bp_floats = 0;
   } else {
     pl.nct_pad = (int)(ceil_div(nct, v.ct) * v.ct);
@@ -194,8 +200,13 @@ void heuristic_sgemm(const GemmKey &k, hmlSgemmVariant *v) {
     v->rt = gv.lpr; v->ct = gv.rb; v->ksplit = gv.split; v->wpb = gv.wpb;
     return;
   }
-  if (k.N > 128 && k.M >= 256 && k.K >= 64) {
-    // Not skinny: plain FP32 dense tiles (A re-read N/128 times otherwise).
+  if (k.N > 128 && k.M >= 256 && k.K >= 64 && (k.N > 512 || k.K <= 512) &&
+      ceil_div(k.M, 128) * ceil_div(k.N, 128) >= 256) {
+    // Not skinny: plain FP32 dense tiles (A re-read N/128 times otherwise),
+    // when its 128 x 128 output tiles alone fill the CUs (it has no K
+    // split).  Measured on MI355X: up to N = 512 the K-split skinny kernels
+    // are faster once K exceeds 512 (up to 2x), and dense wins only barely
+    // below that.
     *v = hmlSgemmVariant{HML_SGEMM_FAMILY_DENSE, 1, 1, 1, 4};
     return;
   }
@@ -242,6 +253,9 @@ void basic_sgemm(const GemmKey &k, hmlSgemmVariant *v) {
 }
 
 bool sgemm_variant_runnable(const GemmKey &k, const hmlSgemmVariant &v) {
+  if (v.family == HML_SGEMM_FAMILY_DOTB)
+    return k.form == kFormDot && v.ksplit >= 1 && v.ksplit <= 4096 && v.wpb == 4 &&
+           skinny_dotb_runnable(v.rt, v.ct);
   if (v.family == HML_SGEMM_FAMILY_DOT)
     return k.form == kFormDot && v.ksplit >= 1 && v.ksplit <= 4096 &&
            (v.wpb == 1 || v.wpb == 2 || v.wpb == 4 || v.wpb == 8) &&
@@ -327,7 +341,9 @@ hmlStatus run_sgemm(const GemmCanon &g, const GemmKey &k,
     p.kchunk = pl.kchunk;
     p.kgroups = pl.ksplit;
     p.alpha = alpha; p.beta = beta; p.read_c = read_c;
-    hipError_t e = launch_skinny_dot(v.rt, v.ct, g.sBk == 1, pl.kw, p, st);
+    hipError_t e = v.family == HML_SGEMM_FAMILY_DOTB
+                       ? launch_skinny_dotb(v.rt, v.ct, g.sBk == 1, p, st)
+                       : launch_skinny_dot(v.rt, v.ct, g.sBk == 1, pl.kw, p, st);
     if (e != hipSuccess) return from_hip(e, "skinny kernel launch");
     if (pl.ksplit > 1) {
       e = launch_reduce(p.P, pl.ksplit, g.M * g.N, g.N, g.C, g.sCm, g.sCn, alpha,

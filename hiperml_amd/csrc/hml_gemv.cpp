@@ -71,27 +71,44 @@ int ilog2_floor(int64_t x) {
 }
 
 void heuristic_sgemv(const GemvKey &k, hmlSgemvVariant *v) {
+  // Fitted to steady-state sweeps of every candidate on MI355X (16384^2,
+  // 8192^2, 65536 x 1023, 2^20 x 64, both forms): 128-bit loads, thin waves
+  // and many of them in flight.
   v->family = k.family;
-  v->wpb = 4;
   if (k.family == HML_SGEMV_FAMILY_DOT) {
-    v->lpr = lpr_for(k.reduced);
+    // Lanes per row: one 128-bit load per lane up to 16 floats, two at 64,
+    // four from 128; rows of 512 .. 2047 floats on 32 lanes, longer on 64.
+    const int64_t loads = std::min<int64_t>(4, std::max<int64_t>(1, k.reduced / 32));
+    v->lpr = lpr_for(ceil_div(k.reduced, loads));
+    if (k.reduced >= 512 && k.reduced < 2048) v->lpr = 32;
+    v->wpb = 2;
     const int G = 64 / v->lpr;
-    v->rb = v->lpr == 64 ? 4 : (G >= 4 ? 4 : 4);
+    v->rb = v->lpr >= 32 ? (k.outputs >= 8192 ? 2 : 1)
+                         : (v->lpr <= 8 && k.outputs >= ((int64_t)1 << 17) ? 4 : 1);
+    // Split the rows into enough pieces for about 64K waves when A is at
+    // least 512 MB, 4K below that (a split adds partials and a reduction
+    // pass, which costs more than it hides on mid-size problems), each
+    // piece at least 4 loads per lane long.
     const int64_t rows_per_wave = (int64_t)G * v->rb;
     const int64_t waves = ceil_div(k.outputs, rows_per_wave);
+    const int64_t target = (double)k.outputs * (double)k.reduced * 4.0 >= 536870912.0 ? 65536 : 4096;
     int64_t split = 1;
-    if (waves < 2048) {
-      split = ceil_div(2048, waves);
-      split = std::min<int64_t>(split, std::max<int64_t>(1, k.reduced / (4 * v->lpr * 8)));
+    if (waves < target) {
+      split = ceil_div(target, waves);
+      split = std::min<int64_t>(split, std::max<int64_t>(1, k.reduced / (4 * v->lpr * 4)));
     }
     v->split = (int)std::max<int64_t>(1, std::min<int64_t>(split, 1024));
   } else {
-    v->lpr = lpr_for(k.outputs);
+    // Column chunks of at most 128 floats (32 lanes); the rows split so
+    // that about 1024 workgroups of 4 waves run, each split at least 64
+    // rows long, at most 512 splits.
+    v->lpr = std::min(32, lpr_for(k.outputs));
     v->rb = 1;
+    v->wpb = 4;
     const int64_t ncb = ceil_div(k.outputs, 4 * v->lpr);
     int64_t split = ceil_div(1024, ncb);
-    split = std::min<int64_t>(split, std::max<int64_t>(1, k.reduced / 256));
-    v->split = (int)std::max<int64_t>(1, std::min<int64_t>(split, 4096));
+    split = std::min<int64_t>(split, std::max<int64_t>(1, k.reduced / 64));
+    v->split = (int)std::max<int64_t>(1, std::min<int64_t>(split, 512));
   }
 }
 

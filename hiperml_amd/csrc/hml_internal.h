@@ -86,6 +86,10 @@ bool skinny_dot_runnable(int rt, int ct, int kw);
 int skinny_dot_resident_waves(int rt, int ct);
 size_t skinny_dot_lds_bytes(int rt, int ct, int kw);
 // bvec: B is K-contiguous (sBk == 1), read with 16-byte loads
+// k_skinny_dotb: 4 waves on 4 row tiles share each B stage through LDS;
+// a K split spans workgroups (kgroups > 1 writes partials).
+bool skinny_dotb_runnable(int rt, int ct);
+hipError_t launch_skinny_dotb(int rt, int ct, bool bvec, const DotParams &p, hipStream_t st);
 hipError_t launch_skinny_dot(int rt, int ct, bool bvec, int kw, const DotParams &p,
                              hipStream_t st);
 
@@ -159,4 +163,19 @@ void grouped_class_shape(int cls, int *rt, int *ct);
 int grouped_class_blocks(int cls);
 hipError_t launch_grouped(int cls, const GroupProblem *probs, const GroupTile *tiles,
                           int64_t ntiles, int nblocks, hipStream_t st);
+// k-means assignment (hml_k_kmeans.hip).  kKmWaves waves per workgroup,
+// each on its own 16-row tile; NI = dims / 16 rounded up, NT centroid tiles
+// of 16 per screening pass, KC centroids per LDS chunk.
+constexpr int kKmWaves = 4;
+constexpr size_t kKmLdsBytes = 65536;
+struct KmeansPlan {
+  int ni = 1, nt = 1, KC = 0;
+  size_t lds = 0;
+  int64_t tiles = 0;
+};
+KmeansPlan plan_kmeans(int64_t nrows, int D, int K);
+hipError_t launch_kmeans_assign(const KmeansPlan &pl, uint32_t *assign, const float *rows,
+                                int64_t nrows, int D, const float *ctrds, int K, bool fused,
+                                hipStream_t st);
+
 }  // namespace hml

@@ -21,11 +21,10 @@
 // relies on texture clamping, include/hml_sgemv_kernel_template.h:193-211):
 // every access is bounds-checked.
 #include "hml_internal.h"
+#include "hml_kdev.h"
 
 namespace hml {
 namespace {
-
-typedef float f4 __attribute__((ext_vector_type(4)));
 
 template <int CTRL>
 __device__ __forceinline__ float dpp(float v) {
@@ -408,6 +407,252 @@ __global__ __launch_bounds__(256) void k_gemv_axpy_stream(GemvParams p) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// The same two pipelines for rows at any 4-byte alignment (odd lda, odd
+// contiguous extent, unaligned A or x): 16-byte buffer loads need only dword
+// alignment.  The descriptors are wave-uniform; every lane-dependent part of
+// an address is a 32-bit lane offset, and a lane offset of kOOB (rows past
+// the end, elements past the chunk or past N) reads zero without touching
+// memory.  The host launches these only when every in-range lane offset is
+// below 2^31 (gemv_buf_fits).
+// ---------------------------------------------------------------------------
+template <int LPR, int RB>
+__global__ __launch_bounds__(256) void k_gemv_dot_buf(GemvParams p) {
+  constexpr int G = 64 / LPR;
+  constexpr int P = RB <= 2 ? 4 : 2;
+  constexpr int STEP = 4 * LPR;
+  const int lane = threadIdx.x & 63;
+  const int wpb = blockDim.x >> 6;
+  const int64_t task = (int64_t)blockIdx.x * wpb +
+                       __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t rows_per_wave = (int64_t)G * RB;
+  const int64_t nrb = (p.M + rows_per_wave - 1) / rows_per_wave;
+  const int64_t rb = task / p.split;
+  const int sp = (int)(task - rb * p.split);
+  if (rb >= nrb) return;  // no barriers in this kernel
+  const int grp = lane / LPR, q = lane % LPR;
+
+  const int64_t cb = (int64_t)sp * p.chunk;
+  int64_t ce = cb + p.chunk;
+  if (ce > p.N) ce = p.N;
+  const int64_t nfull = ce > cb ? (ce - cb) / STEP : 0;
+  const int64_t row0 = rb * rows_per_wave;
+
+  const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.A + row0 * p.lda + cb, kRsrcMax);
+  const __amdgpu_buffer_rsrc_t rx = make_rsrc(p.x + cb, kRsrcMax);
+  uint32_t aoff[RB];  // row row0 + iG + grp, column cb + 4q
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+    aoff[i] = row0 + i * G + grp < p.M
+                  ? (uint32_t)((((int64_t)i * G + grp) * p.lda + 4 * q) * 4) : kOOB;
+  const uint32_t xoff = (uint32_t)(16 * q);
+
+  float acc[RB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i) acc[i] = 0.f;
+
+  f4 bx[P], ba[P][RB];
+  auto load = [&](int j, int64_t s) {
+    const int32_t so = (int32_t)(s * STEP * 4);
+    bx[j] = buf_load4<0>(rx, xoff, so);
+#pragma unroll
+    for (int i = 0; i < RB; ++i) ba[j][i] = buf_load4<kBufNT>(ra, aoff[i], so);
+  };
+  auto comp = [&](int j) {
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      acc[i] += ba[j][i][0] * bx[j][0];
+      acc[i] += ba[j][i][1] * bx[j][1];
+      acc[i] += ba[j][i][2] * bx[j][2];
+      acc[i] += ba[j][i][3] * bx[j][3];
+    }
+  };
+  if (nfull >= P) {
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      load(j, j);
+      __builtin_amdgcn_sched_barrier(0);  // issue the stages in order
+    }
+    int64_t s = 0;
+    for (; s + 2 * P <= nfull; s += P) {
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        comp(j);
+        load(j, s + P + j);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      comp(j);
+      if (s + P + j < nfull) load(j, s + P + j);
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j)
+      if (s + P + j < nfull) comp(j);
+  } else {
+    for (int64_t s = 0; s < nfull; ++s) {
+      load(0, s);
+      comp(0);
+    }
+  }
+  if (cb + nfull * STEP < ce) {
+    // last partial step, element by element (past ce: out-of-range offset)
+    const int64_t c = nfull * STEP + 4 * q;  // relative to cb
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const bool in = cb + c + e < ce;
+      const float xv = buf_load1(rx, in ? (uint32_t)((c + e) * 4) : kOOB, 0);
+#pragma unroll
+      for (int i = 0; i < RB; ++i) {
+        const uint32_t o = in && aoff[i] != kOOB ? aoff[i] + (uint32_t)((nfull * STEP + e) * 4) : kOOB;
+        acc[i] += buf_load1(ra, o, 0) * xv;
+      }
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < RB; ++i) {
+    const float s = group_sum<LPR>(acc[i]);
+    const int64_t row = row0 + i * G + grp;
+    if (q == 0 && row < p.M) {
+      if (p.split == 1) {
+        float *yp = p.y + row;
+        float v = p.alpha * s;
+        if (p.read_c) v = p.alpha * s + p.beta * (*yp);
+        *yp = v;
+      } else {
+        p.P[(int64_t)sp * p.M + row] = s;
+      }
+    }
+  }
+}
+
+template <int LPR>
+__global__ __launch_bounds__(256) void k_gemv_axpy_buf(GemvParams p) {
+  constexpr int G = 64 / LPR;
+  constexpr int CW = 4 * LPR;
+  constexpr int P = 4;
+  __shared__ f4 red[4][LPR];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wpb = blockDim.x >> 6;
+  const int grp = lane / LPR, q = lane % LPR;
+  const int64_t cblk = (int64_t)blockIdx.x * CW;
+  const int64_t c0 = cblk + 4 * q;
+  const int sp = blockIdx.y;
+  const int64_t rb = (int64_t)sp * p.chunk;
+  int64_t re = rb + p.chunk;
+  if (re > p.M) re = p.M;
+  const int64_t rstep = (int64_t)wpb * G;
+  const int64_t rw = rb + (int64_t)w * G;  // the wave's first row (uniform)
+  const int64_t first_last = rw + (G - 1);
+  const int64_t nfull = re > first_last ? (re - first_last - 1) / rstep + 1 : 0;
+
+  // lane offsets of row rw + grp (+ s rstep through the descriptor base),
+  // columns c0 .. c0 + 3; a group that crosses N loads element by element
+  const int64_t ncol = p.N - c0;
+  const bool whole = ncol >= 4;
+  const uint32_t aoff = (uint32_t)(((int64_t)grp * p.lda + 4 * q) * 4);
+  const uint32_t xoff = (uint32_t)(4 * grp);
+  const __amdgpu_buffer_rsrc_t rx = make_rsrc(p.x + rw, kRsrcMax);
+
+  f4 acc = {0.f, 0.f, 0.f, 0.f};
+  f4 ba[P];
+  float bx[P];
+  auto loadrow = [&](int64_t s, bool in) -> f4 {
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(p.A + (rw + s * rstep) * p.lda + cblk, kRsrcMax);
+    if (whole) return buf_load4<kBufNT>(rs, in ? aoff : kOOB, 0);
+    f4 v;
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      v[e] = buf_load1(rs, in && ncol > e ? aoff + 4 * e : kOOB, 0);
+    return v;
+  };
+  auto load = [&](int j, int64_t s) {
+    bx[j] = buf_load1(rx, xoff, (int32_t)(s * rstep * 4));
+    ba[j] = loadrow(s, true);
+  };
+  auto comp = [&](int j) { acc += ba[j] * bx[j]; };
+  if (nfull >= P) {
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      load(j, j);
+      __builtin_amdgcn_sched_barrier(0);  // issue the stages in order
+    }
+    int64_t s = 0;
+    for (; s + 2 * P <= nfull; s += P) {
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        comp(j);
+        load(j, s + P + j);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      comp(j);
+      if (s + P + j < nfull) load(j, s + P + j);
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j)
+      if (s + P + j < nfull) comp(j);
+  } else {
+    for (int64_t s = 0; s < nfull; ++s) {
+      load(0, s);
+      comp(0);
+    }
+  }
+  {
+    // rows after the whole steps (at most one partial step per lane group)
+    const bool in = rw + nfull * rstep + grp < re;
+    if (rw + nfull * rstep < re) {
+      const float xr = buf_load1(rx, in ? xoff : kOOB, (int32_t)(nfull * rstep * 4));
+      acc += loadrow(nfull, in) * xr;
+    }
+  }
+#pragma unroll
+  for (int off = LPR; off < 64; off <<= 1) {
+    acc[0] += __shfl_xor(acc[0], off);
+    acc[1] += __shfl_xor(acc[1], off);
+    acc[2] += __shfl_xor(acc[2], off);
+    acc[3] += __shfl_xor(acc[3], off);
+  }
+  if (wpb > 1) {
+    if (grp == 0) red[w][q] = acc;
+    __syncthreads();
+    if (w == 0 && grp == 0) {
+      acc = red[0][q];
+      for (int i = 1; i < wpb; ++i) acc += red[i][q];
+    }
+  }
+  if (w == 0 && grp == 0) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int64_t col = c0 + t;
+      if (col < p.N) {
+        if (p.split == 1) {
+          float *yp = p.y + col;
+          float v = p.alpha * acc[t];
+          if (p.read_c) v = p.alpha * acc[t] + p.beta * (*yp);
+          *yp = v;
+        } else {
+          p.P[(int64_t)sp * p.N + col] = acc[t];
+        }
+      }
+    }
+  }
+}
+
+// Every in-range lane offset of the buffer kernels is below 2^31.
+bool gemv_buf_fits(int family, int lpr, int rb, const GemvParams &p) {
+  const int64_t G = 64 / lpr;
+  const int64_t lim = (int64_t)1 << 31;
+  if (family == 1)  // rows of a wave x lda + the chunk
+    return ((G * rb) * p.lda + p.chunk + 4) * 4 < lim;
+  return (G * p.lda + 4 * lpr + 4) * 4 < lim && (p.M + 4) * 4 < lim;
+}
+
 typedef hipError_t (*RunFn)(int, const GemvParams &, hipStream_t);
 
 template <int LPR, int RB, bool VEC>
@@ -418,6 +663,9 @@ hipError_t run_dot(int wpb, const GemvParams &p, hipStream_t st) {
   if (nblk > 0x7fffffff) return hipErrorInvalidConfiguration;
   if (VEC)
     hipLaunchKernelGGL((k_gemv_dot_stream<LPR, RB>), dim3((unsigned)nblk),
+                       dim3(64 * wpb), 0, st, p);
+  else if (gemv_buf_fits(1, LPR, RB, p))
+    hipLaunchKernelGGL((k_gemv_dot_buf<LPR, RB>), dim3((unsigned)nblk),
                        dim3(64 * wpb), 0, st, p);
   else
     hipLaunchKernelGGL((k_gemv_dot<LPR, RB, false>), dim3((unsigned)nblk),
@@ -431,6 +679,9 @@ hipError_t run_axpy(int wpb, const GemvParams &p, hipStream_t st) {
   if (ncb > 0x7fffffff || p.split > 65535) return hipErrorInvalidConfiguration;
   if (VEC)
     hipLaunchKernelGGL((k_gemv_axpy_stream<LPR>), dim3((unsigned)ncb, (unsigned)p.split),
+                       dim3(64 * wpb), 0, st, p);
+  else if (gemv_buf_fits(2, LPR, 1, p))
+    hipLaunchKernelGGL((k_gemv_axpy_buf<LPR>), dim3((unsigned)ncb, (unsigned)p.split),
                        dim3(64 * wpb), 0, st, p);
   else
     hipLaunchKernelGGL((k_gemv_axpy<LPR, false>), dim3((unsigned)ncb, (unsigned)p.split),

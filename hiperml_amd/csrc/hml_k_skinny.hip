@@ -25,11 +25,10 @@
 #include <mutex>
 
 #include "hml_internal.h"
+#include "hml_kdev.h"
 
 namespace hml {
 namespace {
-
-typedef float f4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ f4 mfma4(float a, float b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -105,28 +104,6 @@ constexpr size_t kDotMaxLds = 65536;
 // A tile address (floats) of row x, 16-byte chunk kc
 __device__ __forceinline__ int dot_tile_off(int x, int kc) {
   return x * kDotStageK + 4 * (kc ^ (x & 15));
-}
-
-// Raw buffer descriptor (gfx9 format word) of `bytes` bytes at `base`.  A
-// load whose lane offset is >= the size returns zero and reads nothing.
-constexpr int kBufNT = 2;                 // cache policy bit: non-temporal
-constexpr int32_t kRsrcMax = 0x7fffffff;  // size of the operand descriptors
-constexpr uint32_t kOOB = 0x80000000u;    // lane offset past every descriptor
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base,
-                                                           int32_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, bytes, 0x00020000);
-}
-// 16-byte load at any 4-byte-aligned offset: buffer_load_dwordx4 needs only
-// dword alignment on MI355X (checked bit-exact with odd leading dimensions
-// by the parity tests).
-template <int AUX>
-__device__ __forceinline__ f4 buf_load4(__amdgpu_buffer_rsrc_t rs, uint32_t voff,
-                                        int32_t soff) {
-  return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, AUX));
-}
-__device__ __forceinline__ float buf_load1(__amdgpu_buffer_rsrc_t rs, uint32_t voff,
-                                           int32_t soff) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
 }
 
 __device__ __forceinline__ void dot_store(const DotParams &p, int64_t row,
@@ -656,6 +633,225 @@ __global__ void k_fill(float *dst, size_t n, float value) {
   for (; i < n; i += stride) dst[i] = value;
 }
 
+
+// ---------------------------------------------------------------------------
+// DOT form with B shared through LDS: k_skinny_dotb.
+//
+// The 4 waves of a workgroup take 4 consecutive row tiles (16 RT rows
+// each) of one column block and one K chunk, so a single copy of every
+// 64-k stage of B serves all four.  The workgroup loads the stage
+// cooperatively (CT 16-byte loads per thread) and stores it with column n's
+// 64 k values together, 16-byte chunk kq at n*64 + 4*(kq ^ (n % 16)), from
+// where each wave reads its MFMA fragments with conflict-free 16-byte LDS
+// reads.  A goes through each wave's private LDS tile as in k_skinny_dot.
+// Registers hold 2 stages of A and of the thread's share of B (CT loads
+// instead of the 16 CT a wave needs for its own copy), which keeps wide
+// column blocks at two or more waves per SIMD, and B's L2 traffic falls by
+// the 4 waves.  One workgroup barrier per stage; a K split spans workgroups
+// (partials summed in a fixed order by k_reduce).
+// ---------------------------------------------------------------------------
+constexpr int kDotbWaves = 4;
+// LDS bytes of k_skinny_dotb<rt, ct>: 4 A tiles and 2 B stages
+constexpr size_t dotb_lds(int rt, int ct) {
+  return (size_t)(kDotbWaves * rt + 2 * ct) * 16 * kDotStageK * sizeof(float);
+}
+
+__device__ __forceinline__ int dotb_off(int n, int kq) { return n * kDotStageK + 4 * (kq ^ (n & 15)); }
+
+template <int RT, int CT, bool BVEC>
+__global__ __launch_bounds__(64 * kDotbWaves) void k_skinny_dotb(DotParams p) {
+  constexpr int PS = 2;
+  constexpr int NQ = 4 * RT;        // A loads per stage per lane
+  constexpr int TW = RT * 16 * kDotStageK;  // floats of a wave's A tile
+  constexpr int BW = CT * 16 * kDotStageK;  // floats of one B stage
+  extern __shared__ f4 dotb_smem[];
+  float *lds = reinterpret_cast<float *>(dotb_smem);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int r = lane & 15, g = lane >> 4;
+  float *T = lds + w * TW;
+  float *Bs = lds + kDotbWaves * TW;  // [2][16 CT][64]
+  const int64_t rbk = blockIdx.x / p.kgroups;
+  const int kgi = (int)(blockIdx.x - rbk * p.kgroups);
+  const int64_t row0 = (rbk * kDotbWaves + w) * 16 * RT;
+  const int ct0 = blockIdx.y * CT;
+  const int64_t n0 = (int64_t)ct0 * 16;
+  const int64_t kb = (int64_t)kgi * p.kchunk;
+  int64_t ke = kb + p.kchunk;
+  if (ke > p.K) ke = p.K;
+  const int64_t nst = ke > kb ? (ke - kb) / kDotStageK : 0;
+  const bool tail = kb + nst * kDotStageK < ke;
+
+  // A: as in dot_tile_lds (rows past M read zero)
+  const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.A + row0 * p.sAm + kb, kRsrcMax);
+  const __amdgpu_buffer_rsrc_t rnull = make_rsrc(p.A, 0);
+  uint32_t aoff[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+    aoff[q] = row0 + 4 * q + g < p.M ? (uint32_t)(((4 * q + g) * p.sAm + 4 * r) * 4) : kOOB;
+  // B: thread tid's CT 16-byte pieces of a stage.  K-contiguous B: piece
+  // f = tid + 256 i is column n = f / 16, k-quad f % 16.  N-contiguous B:
+  // k = f / (4 CT), columns 4 (f % 4CT) .. +3, stored element by element.
+  const __amdgpu_buffer_rsrc_t rb = make_rsrc(BVEC ? p.B + kb + n0 * p.sBn : p.B + kb * p.sBk + n0, kRsrcMax);
+  uint32_t boff[CT];
+  int bn[CT], bk[CT];
+  bool bwhole[CT];
+#pragma unroll
+  for (int i = 0; i < CT; ++i) {
+    const int f = tid + 256 * i;
+    if (BVEC) {
+      bn[i] = f >> 4;
+      bk[i] = 4 * (f & 15);
+      bwhole[i] = true;
+      boff[i] = n0 + bn[i] < p.N ? (uint32_t)((bn[i] * p.sBn + bk[i]) * 4) : kOOB;
+    } else {
+      bk[i] = f / (4 * CT);
+      bn[i] = 4 * (f % (4 * CT));
+      bwhole[i] = n0 + bn[i] + 3 < p.N;
+      boff[i] = n0 + bn[i] < p.N ? (uint32_t)((bk[i] * p.sBk + bn[i]) * 4) : kOOB;
+    }
+  }
+
+  f4 acc[RT][CT];
+#pragma unroll
+  for (int t = 0; t < RT; ++t)
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[t][c] = f4{0.f, 0.f, 0.f, 0.f};
+  f4 a[PS][NQ];
+  f4 b[PS][CT];
+  auto loadA = [&](int j, int64_t s) {
+    const __amdgpu_buffer_rsrc_t rs = s < nst ? ra : rnull;
+    const int32_t so = (int32_t)(s * kDotStageK * 4);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) a[j][q] = buf_load4<kBufNT>(rs, aoff[q], so);
+  };
+  auto loadB = [&](int j, int64_t s) {
+    const __amdgpu_buffer_rsrc_t rs = s < nst ? rb : rnull;
+    const int32_t so = (int32_t)(s * kDotStageK * (BVEC ? 1 : p.sBk) * 4);
+#pragma unroll
+    for (int i = 0; i < CT; ++i) {
+      if (bwhole[i]) {
+        b[j][i] = buf_load4<0>(rs, boff[i], so);
+      } else {
+        f4 x;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          x[e] = buf_load1(rs, n0 + bn[i] + e < p.N ? boff[i] + 4 * e : kOOB, so);
+        b[j][i] = x;
+      }
+    }
+  };
+  // stage j's registers into LDS (A tile of the wave, B buffer `buf`), then
+  // a workgroup barrier
+  auto put = [&](int j, int buf) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+      *reinterpret_cast<f4 *>(T + dot_tile_off(4 * q + g, r)) = a[j][q];
+    float *B2 = Bs + buf * BW;
+#pragma unroll
+    for (int i = 0; i < CT; ++i) {
+      if (BVEC) {
+        *reinterpret_cast<f4 *>(B2 + dotb_off(bn[i], bk[i] >> 2)) = b[j][i];
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          B2[dotb_off(bn[i] + e, bk[i] >> 2) + (bk[i] & 3)] = b[j][i][e];
+      }
+    }
+    __syncthreads();
+  };
+  auto compute = [&](int buf) {
+    const float *B2 = Bs + buf * BW;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      f4 x[RT][2], y[CT][2];
+#pragma unroll
+      for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int v = 0; v < 2; ++v)
+          x[t][v] = *reinterpret_cast<const f4 *>(T + dot_tile_off(16 * t + r, 8 * u + 4 * v + g));
+#pragma unroll
+      for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int v = 0; v < 2; ++v)
+          y[c][v] = *reinterpret_cast<const f4 *>(B2 + dotb_off(16 * c + r, 8 * u + 4 * v + g));
+#pragma unroll
+      for (int v = 0; v < 2; ++v)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+          for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int c = 0; c < CT; ++c)
+              acc[t][c] = mfma4(x[t][v][jj], y[c][v][jj], acc[t][c]);
+    }
+    // the wave's next put() may overwrite its A tile only after these reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+
+#pragma unroll
+  for (int j = 0; j < PS; ++j) {
+    loadA(j, j);
+    loadB(j, j);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  // nst and the tail are the same for the whole workgroup: the barriers in
+  // put() are reached by every wave
+  for (int64_t s = 0; s < nst; s += PS) {
+#pragma unroll
+    for (int j = 0; j < PS; ++j) {
+      const bool live = s + j < nst;
+      if (live) put(j, (int)((s + j) & 1));
+      loadA(j, s + PS + j);
+      loadB(j, s + PS + j);
+      if (live) compute((int)((s + j) & 1));
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  if (tail) {
+    // partial last stage, element by element past ke
+    const int64_t k0 = nst * kDotStageK;  // relative to kb
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      f4 x;
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) {
+        const uint32_t vo = (aoff[q] == kOOB || kb + k0 + 4 * r + jj >= ke) ? kOOB
+                            : aoff[q] + (uint32_t)((k0 + jj) * 4);
+        x[jj] = buf_load1(ra, vo, 0);
+      }
+      a[0][q] = x;
+    }
+#pragma unroll
+    for (int i = 0; i < CT; ++i) {
+      f4 x;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t k = BVEC ? k0 + bk[i] + e : k0 + bk[i];
+        const int64_t n = n0 + bn[i] + (BVEC ? 0 : e);
+        uint32_t vo = kOOB;
+        if (n < p.N && kb + k < ke)
+          vo = BVEC ? (uint32_t)((bn[i] * p.sBn + k) * 4) : (uint32_t)((k * p.sBk + bn[i] + e) * 4);
+        x[e] = buf_load1(rb, vo, 0);
+      }
+      b[0][i] = x;
+    }
+    put(0, (int)(nst & 1));
+    compute((int)(nst & 1));
+  }
+
+#pragma unroll
+  for (int t = 0; t < RT; ++t)
+#pragma unroll
+    for (int c = 0; c < CT; ++c)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        dot_store(p, row0 + 16 * t + 4 * g + i, (int64_t)(ct0 + c) * 16 + r, kgi, acc[t][c][i]);
+}
+
 template <int RT, int CT, bool BV>
 hipError_t run_dot(int kw, const DotParams &p, hipStream_t st) {
   const int64_t nrb = (p.M + 16 * RT - 1) / (16 * RT);
@@ -687,6 +883,41 @@ DotFn find_dot(int rt, int ct, bool bv) {
     case 36: return dot_fn<2, 4>(bv);
     case 65: return dot_fn<4, 1>(bv);
     case 66: return dot_fn<4, 2>(bv);
+    default: return nullptr;
+  }
+}
+
+
+template <int RT, int CT, bool BV>
+hipError_t run_dotb(const DotParams &p, hipStream_t st) {
+  const int64_t nrb = (p.M + kDotbWaves * 16 * RT - 1) / (kDotbWaves * 16 * RT);
+  const int64_t nct = (p.N + 15) / 16;
+  const int64_t ncb = (nct + CT - 1) / CT;
+  const int64_t nblk = nrb * p.kgroups;
+  if (nblk > 0x7fffffff || ncb > 65535) return hipErrorInvalidConfiguration;
+  const size_t lds = dotb_lds(RT, CT);
+  hipLaunchKernelGGL((k_skinny_dotb<RT, CT, BV>), dim3((unsigned)nblk, (unsigned)ncb),
+                     dim3(64 * kDotbWaves), lds, st, p);
+  return hipGetLastError();
+}
+
+typedef hipError_t (*DotbFn)(const DotParams &, hipStream_t);
+
+template <int RT, int CT>
+DotbFn dotb_fn(bool bv) {
+  return bv ? run_dotb<RT, CT, true> : run_dotb<RT, CT, false>;
+}
+
+// instantiated (rt, ct): rt in {1, 2}, ct in {1, 2, 4} (4 row tiles of 4
+// take more than 64 KB of LDS)
+DotbFn find_dotb(int rt, int ct, bool bv) {
+  switch (rt * 16 + ct) {
+    case 17: return dotb_fn<1, 1>(bv);
+    case 18: return dotb_fn<1, 2>(bv);
+    case 20: return dotb_fn<1, 4>(bv);
+    case 33: return dotb_fn<2, 1>(bv);
+    case 34: return dotb_fn<2, 2>(bv);
+    case 36: return dotb_fn<2, 4>(bv);
     default: return nullptr;
   }
 }
@@ -792,6 +1023,16 @@ hipError_t launch_skinny_dot(int rt, int ct, bool bvec, int kw, const DotParams 
   DotFn fn = find_dot(rt, ct, bvec);
   return fn(kw, p, st);
 }
+bool skinny_dotb_runnable(int rt, int ct) {
+  return find_dotb(rt, ct, true) != nullptr && dotb_lds(rt, ct) <= kDotMaxLds;
+}
+
+hipError_t launch_skinny_dotb(int rt, int ct, bool bvec, const DotParams &p, hipStream_t st) {
+  if (!skinny_dotb_runnable(rt, ct) || p.kgroups < 1 || p.kchunk % kDotStageK != 0)
+    return hipErrorInvalidValue;
+  return find_dotb(rt, ct, bvec)(p, st);
+}
+
 hipError_t launch_skinny(int form, int rt, int ct, bool vec, int wpb,
                          const SkinnyParams &p, hipStream_t st) {
   RunFn fn = find_skinny(form, rt, ct, vec);

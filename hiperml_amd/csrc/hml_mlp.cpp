@@ -207,10 +207,14 @@ std::once_flag g_autoload_once;
 
 void selector_autoload() {
   std::call_once(g_autoload_once, [] {
+    // Opt-in: HML_SELECTOR=shipped loads the model shipped beside the
+    // library (../config/hml_selector_mi355x.bin), any other non-empty value
+    // other than off / 0 names a model file; unset keeps the table and the
+    // heuristic.
     const char *env = getenv("HML_SELECTOR");
-    if (env && (!strcmp(env, "off") || !strcmp(env, "0"))) return;
+    if (!env || !*env || !strcmp(env, "off") || !strcmp(env, "0")) return;
     std::string path;
-    if (env && *env) {
+    if (strcmp(env, "shipped") != 0) {
       path = env;
     } else {
       Dl_info info;

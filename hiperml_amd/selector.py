@@ -2,7 +2,9 @@
 
 The pipeline (all timings and all training run on the GPU through libhml):
   1. random_shapes(): M, K log-uniform in [16, 65536], N log-uniform in
-     [1, 512], random transA/transB/beta==0, rejecting problems whose
+     [1, 512]; a third of the M, K and N draws rounded to a power of two
+     and (M, K) another third to a multiple of 64, since real problems are
+     mostly aligned; random transA/transB/beta==0, rejecting problems whose
      autotune working set exceeds the memory cap;
   2. autotune(): the exhaustive, validating autotuner (hmlSgemmAutotune)
      times every candidate variant of each shape into a CSV database;
@@ -34,6 +36,17 @@ def autotune_bytes(M, N, K):
     return 4.0 * (M * K + K * N + 3 * M * N) + 4.0 * (K * max(N, 16) * 2 + 64 * M * N)
 
 
+def _round_dim(rng, x, mult):
+    """A third of the draws to the nearest power of two, a third (when
+    mult > 1) to a multiple of `mult`, the rest unchanged."""
+    u = rng.uniform()
+    if u < 1.0 / 3.0:
+        return 1 << max(0, int(round(math.log2(x))))
+    if u < 2.0 / 3.0 and mult > 1 and x >= mult:
+        return int(round(x / mult)) * mult
+    return x
+
+
 def random_shapes(n, seed, mem_cap=8e9, lo=16, hi=65536, nlo=1, nhi=512):
     rng = np.random.default_rng(seed)
     out = []
@@ -41,11 +54,16 @@ def random_shapes(n, seed, mem_cap=8e9, lo=16, hi=65536, nlo=1, nhi=512):
         M = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
         K = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
         N = int(round(math.exp(rng.uniform(math.log(nlo), math.log(nhi)))))
+        M, K, N = _round_dim(rng, M, 64), _round_dim(rng, K, 64), _round_dim(rng, N, 1)
         if autotune_bytes(M, N, K) > mem_cap:
             continue
         ta, tb, b0 = (int(x) for x in rng.integers(0, 2, 3))
         out.append((M, N, K, ta, tb, b0))
     return out
+
+
+# Row-major 8192 x N x 8192 problems, beta = 0: the skinny-SGEMM sweep.
+NAMED_SHAPES = [(8192, n, 8192, 0, 0, 1) for n in (1, 2, 4, 8, 16, 32, 64, 128)]
 
 
 def autotune(shapes, db_path, trials=3, flush=True, seed=1, progress=None):
@@ -178,7 +196,7 @@ def install_table(db_train):
 
 
 def run(out_dir, n_train, n_eval, seed=2024, epochs=300, trials=3, mem_cap=8e9,
-        train_db=None, eval_db=None):
+        train_db=None, eval_db=None, named=True):
     """End-to-end: autotune random training and held-out shapes (unless
     databases are given), train the MLP on the GPU, evaluate, save the model
     (hml_selector_mi355x.bin) and a JSON report into out_dir."""
@@ -198,6 +216,12 @@ def run(out_dir, n_train, n_eval, seed=2024, epochs=300, trials=3, mem_cap=8e9,
         autotune(shapes, eval_db, trials=trials, seed=seed + 100000,
                  progress=max(1, min(50, n_eval // 10)))
         print("eval autotune done in %.0f s" % (time.time() - t0), flush=True)
+    named_db = None
+    if named:
+        named_db = os.path.join(out_dir, "autotune_named.csv")
+        if os.path.exists(named_db):
+            os.remove(named_db)
+        autotune(NAMED_SHAPES, named_db, trials=trials, seed=seed + 200000)
     dbt, dbe = load_db(train_db), load_db(eval_db)
     keys, X, T, mask, _ = dataset(dbt)
     timings = sum(len(v) for v in dbt.values())
@@ -208,6 +232,7 @@ def run(out_dir, n_train, n_eval, seed=2024, epochs=300, trials=3, mem_cap=8e9,
            "eval_shapes": len(dbe), "epochs": epochs,
            "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
            "fraction_of_autotune_best": summarize(fr),
+           "named_shapes": named_report(net, load_db(named_db), dbt) if named_db else None,
            "seconds": round(time.time() - t0, 1)}
     net.save(os.path.join(out_dir, "hml_selector_mi355x.bin"))
     with open(os.path.join(out_dir, "selector_report.json"), "w") as f:
@@ -228,9 +253,26 @@ def main(argv=None):
     ap.add_argument("--mem-cap", type=float, default=8e9)
     ap.add_argument("--train-db")
     ap.add_argument("--eval-db")
+    ap.add_argument("--no-named", action="store_true",
+                    help="skip the 8192 x N x 8192 shape report")
     a = ap.parse_args(argv)
+    # the pipeline times the heuristic and the table itself: keep any
+    # shipped selector out of this process's selections
+    import os
+    os.environ["HML_SELECTOR"] = "off"
     check(lib.hmlInit())
-    run(a.out, a.train, a.eval, a.seed, a.epochs, a.trials, a.mem_cap, a.train_db, a.eval_db)
+    run(a.out, a.train, a.eval, a.seed, a.epochs, a.trials, a.mem_cap, a.train_db, a.eval_db,
+        not a.no_named)
+
+
+def named_report(net, db_named, db_train):
+    """Per named shape: the fraction of the autotune-best time reached by
+    each selector."""
+    out = {}
+    for key in sorted(db_named):
+        fr = evaluate(net, {key: db_named[key]}, db_train)
+        out["%dx%dx%d" % key[:3]] = {k: round(float(v[0]), 4) for k, v in fr.items() if len(v)}
+    return out
 
 
 def summarize(fracs):

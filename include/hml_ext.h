@@ -54,16 +54,20 @@ typedef enum {
   HML_SGEMM_FAMILY_GEMV   = 3, /* N == 1 routed to the GEMV kernels        */
   HML_SGEMM_FAMILY_BF16X3 = 4, /* split-precision bf16 MFMA, 2-term split  */
   HML_SGEMM_FAMILY_DENSE  = 5, /* plain FP32 MFMA dense tile kernel        */
-  HML_SGEMM_FAMILY_BF16X6 = 6  /* split-precision bf16 MFMA, 3-term split  */
+  HML_SGEMM_FAMILY_BF16X6 = 6, /* split-precision bf16 MFMA, 3-term split  */
+  HML_SGEMM_FAMILY_DOTB   = 7  /* MFMA f32, K-contiguous A, B shared by the
+                                  workgroup through LDS                   */
 } hmlSgemmFamily;
 
 /* Skinny variants (DOT, AXPY): a wave computes rt row tiles (16 rows DOT,
  * 64 rows AXPY) x ct column tiles of 16 over one of ksplit K chunks.  DOT:
  * the wpb (1, 2, 4 or 8) waves of a workgroup take consecutive chunks of one
  * tile and sum them in LDS; chunks beyond one workgroup are summed by a
- * reduction pass; ct <= 4 and rt * ct <= 8.  AXPY: wpb (1, 2 or 4) waves per
- * workgroup take different tiles; every K split is summed by a reduction
- * pass. */
+ * reduction pass; ct <= 4 and rt * ct <= 8.  DOTB: the 4 waves (wpb = 4)
+ * of a workgroup take 4 consecutive row tiles and share each stage of B
+ * through LDS; every K chunk is a workgroup and splits are summed by a
+ * reduction pass; rt in {1, 2}, ct in {1, 2, 4}.  AXPY: wpb (1, 2 or 4) waves per workgroup take
+ * different tiles; every K split is summed by a reduction pass. */
 typedef struct {
   int32_t family; /* hmlSgemmFamily                                          */
   int32_t rt;     /* row tiles per wave                                       */

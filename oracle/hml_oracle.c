@@ -24,6 +24,23 @@
  *      accumulation of the FP32 inputs, plus the |A||B| product used for the
  *      Higham gamma_K error bound.
  *   5. Flop formulas <- include/hml_flops.h:33-45,186,202.
+ *   6. k-means nearest-centroid assignment:
+ *        oracle_kmeans_assign <- the assign kernels
+ *          hmlKmeansAssignCtrdSmem / SmemUnroll / Gmem
+ *          (include/hml_kmeans_kernel_template.h:18-88, 90-316, 323-367)
+ *          and the CPU path hmlKmeansAssign / hmlKmeansDistance
+ *          (src/hml_kmeans_cpu.cu:55-95).
+ *      Per row, per centroid k ascending: dist = sum over dims ascending of
+ *      delta*delta, delta = x - c; the row takes the first k whose dist is
+ *      strictly below every earlier one, starting from FLT_MAX.  `fused`
+ *      selects the GPU kernels' arithmetic (nvcc contracts
+ *      `dist += delta*delta` into one fused multiply-add by default, and the
+ *      reference sets no -fmad flag, CMakeLists.txt) or the CPU path's
+ *      (product and sum rounded separately, baseline x86-64).  A row with no
+ *      dist below FLT_MAX (NaN or overflowing data) gets numClusts: the CPU
+ *      path's value for its first row (it leaves minK over from the previous
+ *      row afterwards, src/hml_kmeans_cpu.cu:83-93), the GPU kernels' is
+ *      undefined (minK uninitialised).
  *
  * Pinning (what checks each function, besides retyping):
  *   - sgemm/sgemv_cpu: integer-valued inputs give exact results that are
@@ -37,7 +54,14 @@
  *   - csum: exact on integer data; equals math.fsum within 1 ulp on random.
  *   - hml_random_fill: the first values of glibc rand() after srand(1) are a
  *     published constant sequence (1804289383, 846930886, ...).
+ *   - kmeans_assign: closed forms (centroids equal to rows: distance 0, the
+ *     first equal centroid wins ties), brute-force agreement with a float64
+ *     argmin on separated data, and constructed inputs on which the fused
+ *     and the separately rounded arithmetic give different assignments whose
+ *     expected values follow from exact float64 arithmetic.  See
+ *     tests/test_oracle.py.
  */
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -176,3 +200,34 @@ void oracle_gemm_f64(double *C64, double *absC64, const float *A,
 double oracle_flops_sgemm(double M, double N, double K) { return 2.0 * M * N * K; }
 /* include/hml_flops.h:33-34,186 : (MN + 2M) + MN */
 double oracle_flops_sgemv(double M, double N) { return 2.0 * M * N + 2.0 * M; }
+
+/* ---- 6. k-means assignment ---------------------------------------------- */
+
+/* rows: numRows x numDims, ctrds: numClusts x numDims, both row-major. */
+void oracle_kmeans_assign(uint32_t *out, const float *rows, long numRows,
+                          int numDims, const float *ctrds, int numClusts,
+                          int fused) {
+  for (long r = 0; r < numRows; ++r) {
+    const float *x = rows + r * (long)numDims;
+    float minDist = FLT_MAX;
+    uint32_t minK = (uint32_t)numClusts;
+    for (int k = 0; k < numClusts; ++k) {
+      const float *c = ctrds + (long)k * numDims;
+      float dist = 0.0f;
+      for (int d = 0; d < numDims; ++d) {
+        const float delta = x[d] - c[d];
+        if (fused) {
+          dist = fmaf(delta, delta, dist);
+        } else {
+          const float sq = delta * delta;
+          dist = dist + sq;
+        }
+      }
+      if (dist < minDist) {
+        minDist = dist;
+        minK = (uint32_t)k;
+      }
+    }
+    out[r] = minK;
+  }
+}

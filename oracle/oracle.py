@@ -11,6 +11,9 @@ Functions and the reference code they restate:
               apps/gemv/hml_sgemv_main.cu:436-446, src/hml_utils.cu:60
   gemm_f64    FP64 oracle (double accumulation, any strided layout)
   flops_*     include/hml_flops.h:33-45,186,202
+  kmeans_assign  include/hml_kmeans_kernel_template.h:18-88,90-316,323-367
+              (GPU assign kernels, fused=True), src/hml_kmeans_cpu.cu:55-95
+              (CPU path, fused=False)
 """
 import ctypes
 import os
@@ -49,6 +52,7 @@ def _load():
     lib.oracle_flops_sgemm.restype = ctypes.c_double
     lib.oracle_flops_sgemv.argtypes = [ctypes.c_double] * 2
     lib.oracle_flops_sgemv.restype = ctypes.c_double
+    lib.oracle_kmeans_assign.argtypes = [ctypes.POINTER(ctypes.c_uint32), fp, l, i, fp, i, i]
     return lib
 
 
@@ -149,3 +153,18 @@ def layout_strides(order, trans, ld):
     stored in `order` ('row'/'col') with leading dimension ld."""
     rowview = (order == "row") == (not trans)
     return (ld, 1) if rowview else (1, ld)
+
+
+def kmeans_assign(rows, ctrds, fused=True):
+    """uint32 nearest-centroid index per row (numClusts where no distance is
+    below FLT_MAX), in the reference kernels' arithmetic (fused=True) or the
+    reference CPU path's (fused=False)."""
+    rows = np.ascontiguousarray(rows, np.float32)
+    ctrds = np.ascontiguousarray(ctrds, np.float32)
+    n, d = rows.shape
+    k, d2 = ctrds.shape
+    assert d == d2
+    out = np.empty(n, np.uint32)
+    _lib.oracle_kmeans_assign(out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                              _f(rows), n, d, _f(ctrds), k, 1 if fused else 0)
+    return out

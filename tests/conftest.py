@@ -4,9 +4,9 @@ import sys
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-# Selection tests expect the table / heuristic selectors: keep the shipped
-# learned selector out of this process (tests/test_gpu_mlp.py checks it in a
-# child process of its own).
+# Selection tests expect the table / heuristic selectors: keep any learned
+# selector named in the environment out of this process
+# (tests/test_gpu_mlp.py checks loading in child processes of its own).
 os.environ["HML_SELECTOR"] = "off"
 for p in (ROOT, os.path.join(ROOT, "oracle")):
     if p not in sys.path:

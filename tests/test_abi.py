@@ -66,6 +66,15 @@ def test_argument_errors_without_gpu():
     # empty problems succeed without touching memory
     assert hml.lib.hmlSgemmEx(0, 0, 0, 0, 4, 4, 1.0, None, 4, None, 4, 0.0,
                               None, 4, 0, None, 0, None) == 0
+    # k-means assignment: dims 1..64, clusters >= 1, known flags, pointers
+    km = hml.lib.hmlKmeansAssign
+    assert km(1, 1, 10, 0, 1, 4, 0, None) == 1
+    assert km(1, 1, 10, 65, 1, 4, 0, None) == 1
+    assert b"1..64" in hml.lib.hmlLastError()
+    assert km(1, 1, 10, 8, 1, 0, 0, None) == 1
+    assert km(1, 1, 10, 8, 1, 4, 2, None) == 1
+    assert km(None, 1, 10, 8, 1, 4, 0, None) == 1
+    assert km(None, None, 0, 8, None, 4, 1, None) == 0
 
 
 def test_no_torch_or_oracle_in_product():

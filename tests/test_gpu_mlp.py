@@ -119,9 +119,10 @@ def test_selector_install_and_fallback(hml, tmp_path):
 
 
 def test_shipped_selector_autoload(hml, tmp_path):
-    """With HML_SELECTOR naming a model file, the first selection in a fresh
-    process installs it (source 2 = learned); HML_SELECTOR=off keeps the
-    heuristic.  Runs in child processes so this process keeps its state."""
+    """With HML_SELECTOR naming a model file, or `shipped` (the model under
+    config/), the first selection in a fresh process installs it (source 2 =
+    learned); HML_SELECTOR=off or unset keeps the heuristic.  Runs in child
+    processes so this process keeps its state."""
     import subprocess
     import sys
     import os
@@ -131,8 +132,14 @@ def test_shipped_selector_autoload(hml, tmp_path):
     code = ("import sys; sys.path.insert(0, %r); import hiperml_amd as h; "
             "h.check(h.lib.hmlInit()); v, s = h.sgemm_select(1 << 16, 16, 4096); "
             "print('SRC', s)" % os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
-    for setting, want in ((path, 2), ("off", 0)):
-        env = dict(os.environ, HML_SELECTOR=setting)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    shipped = os.path.exists(os.path.join(root, "config", "hml_selector_mi355x.bin"))
+    cases = [(path, 2), ("off", 0), (None, 0)] + ([("shipped", 2)] if shipped else [])
+    for setting, want in cases:
+        env = dict(os.environ)
+        env.pop("HML_SELECTOR", None)
+        if setting is not None:
+            env["HML_SELECTOR"] = setting
         r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
                            text=True, timeout=120)
         print(r.stdout, r.stderr[-1000:])

@@ -98,6 +98,30 @@ def test_integer_exact_every_candidate(hml, case):
         np.testing.assert_array_equal(C, R, err_msg=repr(v))
 
 
+DOTB_SHAPES = [(1000, 64, 700), (4097, 33, 1000), (300, 16, 64), (5000, 128, 2048),
+               (64, 5, 63), (130, 48, 4100), (1, 1, 1), (777, 100, 129)]
+
+
+@pytest.mark.parametrize("tb", [False, True], ids=["BN", "BT"])
+def test_dotb_variants_exact(hml, tb):
+    """The shared-B DOT kernel (family 7) for every instantiated tile shape,
+    B N-contiguous (row-major K x N) and K-contiguous (transposed), one and
+    three K chunks, ragged M / N / K: bit-exact on integer data."""
+    rng = np.random.default_rng(17)
+    for (M, N, K) in DOTB_SHAPES:
+        A = int_matrix(rng, (M, K))
+        B = int_matrix(rng, (K, N))
+        C0 = int_matrix(rng, (M, N))
+        R = exact_product(A, B, 1.0, 1.0, C0)
+        for rt in (1, 2):
+            for ct in (1, 2, 4):
+                for ks in (1, 3):
+                    v = hml.SgemmVariant(7, rt, ct, ks, 4)
+                    print("case", M, N, K, "BT" if tb else "BN", v, flush=True)
+                    C = run_gemm(hml, A, B, C0, 1.0, 1.0, "row", False, tb, variant=v)
+                    np.testing.assert_array_equal(C, R, err_msg=repr(v))
+
+
 SMALL = [1, 2, 3, 5, 8, 13, 16, 17]
 EDGE = [31, 32, 33, 63, 64, 65, 127, 128, 129, 255, 256, 257]
 

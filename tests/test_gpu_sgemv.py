@@ -145,3 +145,34 @@ def test_full_size_vs_fp64(hml, shape, trans):
     print("rel_fro %.3e bound %.3e" % (st["rel_fro"], gamma(nx) * st["bound_fro"]))
     assert st["nonfinite"] == 0
     assert st["max_rel_bound"] <= 2 * gamma(nx + 2)
+
+
+@pytest.mark.parametrize("trans", [False, True], ids=["N", "T"])
+def test_unaligned_views_every_candidate(hml, trans):
+    """A starts one float past a 16-byte boundary and has an odd leading
+    dimension (NaN padding, so any read of it shows up), x starts one float
+    in: the buffer-load kernels for dword-aligned rows.  Ragged tails (N and
+    M not multiples of 4), integer data, bit-exact for every candidate."""
+    rng = np.random.default_rng(11)
+    for (M, N, pad) in ((1003, 517, 3), (70, 2049, 1), (4099, 61, 5)):
+        lda = N + pad
+        A = int_matrix(rng, (M, N))
+        Ap = np.full((M, lda), np.nan, np.float32)
+        Ap[:, :N] = A
+        nx, ny = (M, N) if trans else (N, M)
+        x = int_matrix(rng, (nx,))
+        y0 = int_matrix(rng, (ny,))
+        dA = hml.DeviceBuffer.from_numpy(np.concatenate([[np.float32(7)], Ap.ravel()]))
+        dx = hml.DeviceBuffer.from_numpy(np.concatenate([[np.float32(5)], x]).astype(np.float32))
+        R = exact(A, x, trans, 1.0, 1.0, y0)
+        cands = (hml.SgemvVariant * 512)()
+        n = hml.lib.hmlSgemvCandidates(0, int(trans), M, N, cands, 512)
+        assert 0 < n <= 512
+        for i in [None] + list(range(n)):
+            v = None if i is None else cands[i]
+            print("case", M, N, pad, "T" if trans else "N", v, flush=True)
+            dy = hml.DeviceBuffer.from_numpy(y0)
+            hml.sgemv_ex(M, N, dA.ptr + 4, dx.ptr + 4, dy, 1.0, 1.0, trans=trans,
+                         lda=lda, variant=v)
+            hml.synchronize()
+            np.testing.assert_array_equal(dy.to_numpy(y0.shape), R, err_msg=repr(v))

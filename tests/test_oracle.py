@@ -179,3 +179,66 @@ def test_golden_vectors():
     yr = oracle.sgemv_cpu(y.copy(), Av, x, 1.0, 0.0)
     np.testing.assert_array_equal(yr, g["gemv_y_cpu"])
     np.testing.assert_array_equal(oracle.hml_random(8), g["rand_first8"])
+
+
+# ---- k-means assignment ---------------------------------------------------
+
+def fused_flip_case(seed=3):
+    """2-D inputs on which the fused (GPU kernel) and the separately rounded
+    (CPU path) distance sums differ, built so that the expected assignments
+    follow from exact float64 arithmetic.  Row x = 0; centroid 0 at distance
+    exactly 9 (delta (3, 0)); centroid 1 with delta (d0, d1) whose separately
+    rounded sum U is 9 while the fused sum F is below 9.  Fused: 1 (F < 9);
+    separate: 0 (9 is not below 9)."""
+    rng = np.random.default_rng(seed)
+    f32, f64 = np.float32, np.float64
+    for _ in range(200000):
+        d0 = f32(rng.uniform(1.5, 2.5))
+        sq0 = f32(f64(d0) * f64(d0))                 # one rounding (exact product)
+        d1 = f32(math.sqrt(9.0 - float(sq0)))
+        sq1 = f32(f64(d1) * f64(d1))
+        # both sums are exact in float64 (operands within a few binades)
+        U = f32(f64(sq0) + f64(sq1))
+        F = f32(f64(sq0) + f64(d1) * f64(d1))
+        assert math.fsum([float(sq0), float(f64(d1) * f64(d1))]) == f64(sq0) + f64(d1) * f64(d1)
+        if U == f32(9.0) and F < f32(9.0):
+            rows = np.zeros((1, 2), np.float32)
+            ctrds = np.array([[-3.0, 0.0], [-d0, -d1]], np.float32)
+            return rows, ctrds
+    raise AssertionError("no flip case found")
+
+
+def test_kmeans_assign_fused_vs_separate():
+    rows, ctrds = fused_flip_case()
+    assert oracle.kmeans_assign(rows, ctrds, fused=True).tolist() == [1]
+    assert oracle.kmeans_assign(rows, ctrds, fused=False).tolist() == [0]
+
+
+def test_kmeans_assign_closed_forms():
+    rng = np.random.default_rng(4)
+    D, K = 13, 9
+    ctrds = rng.standard_normal((K, D)).astype(np.float32)
+    ctrds[5] = ctrds[2]                              # duplicate: 2 must win
+    pick = rng.integers(0, K, 500)
+    rows = ctrds[pick]
+    want = np.where(pick == 5, 2, pick)
+    for fused in (True, False):
+        np.testing.assert_array_equal(oracle.kmeans_assign(rows, ctrds, fused), want)
+    # one centroid: everything goes to it; NaN / overflowing rows: none
+    one = oracle.kmeans_assign(rows, ctrds[:1])
+    assert (one == 0).all()
+    bad = np.array([[np.nan] * D, [3e38] * D, [0.0] * D], np.float32)
+    assert oracle.kmeans_assign(bad, ctrds).tolist()[:2] == [K, K]
+
+
+def test_kmeans_assign_vs_float64_argmin():
+    rng = np.random.default_rng(6)
+    for (n, D, K) in ((3000, 64, 40), (2000, 3, 7), (500, 17, 200)):
+        ctrds = (rng.standard_normal((K, D)) * 4).astype(np.float32)
+        rows = (ctrds[rng.integers(0, K, n)] + rng.standard_normal((n, D)).astype(np.float32))
+        d = ((rows[:, None, :].astype(np.float64) - ctrds[None].astype(np.float64)) ** 2).sum(-1)
+        srt = np.sort(d, 1)
+        sep = srt[:, 1] - srt[:, 0] > 1e-4 * srt[:, 1]   # no rounding can flip these
+        assert sep.mean() > 0.9
+        got = oracle.kmeans_assign(rows, ctrds)
+        np.testing.assert_array_equal(got[sep], np.argmin(d, 1)[sep])
