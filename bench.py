@@ -187,7 +187,8 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "kernel": "k_gemv_dot (SGEMV A x, 16384 x 16384)",
+                "traffic": None,
+                "kernel": "k_gemv_dot_stream + its split reduction: one SGEMV A x call, 16384 x 16384",
                 "bytes_per_launch": d[4], "avg_launch_ms": round(d_ms, 5)}
     traffic_file = os.path.join(here, "profiles", "traffic_gemv_dot_16384.json")
     if os.path.exists(traffic_file):
@@ -461,6 +462,7 @@ def traffic_from_pmc(fetch_csv, write_csv, dst=None):
     CALIB_BYTES read and written) and checked against the fill kernel
     (CALIB_BYTES written, nothing read)."""
     res = {}
+    target = None  # (kernel, grid) of the 16384^2 launch: the largest GEMV read
     for name, path, counter in (("fetch", fetch_csv, "FETCH_SIZE"),
                                 ("write", write_csv, "WRITE_SIZE")):
         rows = _pmc_rows(path, counter)
@@ -468,10 +470,14 @@ def traffic_from_pmc(fetch_csv, write_csv, dst=None):
         for kn, g, v in rows:
             groups.setdefault((kn, g), []).append(v)
         gemv = [(np.mean(v), k) for k, v in groups.items() if "gemv_dot" in k[0]]
+        if target is None and gemv:
+            target = max(gemv)[1]
         copy = [(np.mean(v), k) for k, v in groups.items() if "copy" in k[0].lower()]
         fill = [(np.mean(v), k) for k, v in groups.items() if "fill_uniform" in k[0]]
-        res[name] = {"gemv_bytes": max(gemv)[0] if gemv else None,
-                     "gemv_kernel": max(gemv)[1][0] if gemv else None,
+        g_bytes = float(np.mean(groups[target])) if target in groups else None
+        res[name] = {"gemv_bytes": g_bytes,
+                     "gemv_kernel": target[0] if target else None,
+                     "gemv_grid": target[1] if target else None,
                      "copy_bytes": max(copy)[0] if copy else None,
                      "copy_kernel": max(copy)[1][0] if copy else None,
                      "fill_bytes": max(fill)[0] if fill else None}

@@ -5,22 +5,23 @@
 // 19-191, 229-333).  Same operation C = alpha*A*B + beta*C; different
 // machine model:
 //
-//  * The streamed operand A (the big one) is read exactly once from HBM with
-//    128-bit loads straight into registers; no LDS staging, no barriers.
-//    Every wave is an independent work item (a block of rows x a K range).
-//  * The small operand B is packed once per call into the fragment order of
-//    v_mfma_f32_16x16x4f32, so each wave fetches its B fragments with
-//    coalesced 128-bit loads that hit in L1/L2 (B is re-read by every row
-//    block; it is small by construction).
 //  * FP32 products run on the matrix cores (2x the non-packed VALU rate).
 //    v_mfma_f32_16x16x4f32: lane l supplies A[m=l%16][k=l/16] and
 //    B[k=l/16][n=l%16]; the 16x16 result sits in 4 VGPRs, lane l holding
 //    D[4*(l/16)+i][l%16].  The reduction index k is a sum, so any
-//    permutation of K that A and B share is legal; each form picks the
+//    permutation of K that A and B share is legal; each kernel picks the
 //    permutation that makes its A loads contiguous.
-//  * K is split across waves when rows alone cannot fill 256 CUs; the
-//    partial tiles are summed in a fixed order by a second kernel
-//    (deterministic, no atomics).
+//  * The streamed operand A (the big one) is read exactly once from HBM.
+//    DOT form (A contiguous along K): k_skinny_dot loads whole 256-byte row
+//    pieces, transposes them through a per-wave LDS tile and reads B in
+//    place; k_skinny_dotb shares each B stage between the 4 waves of a
+//    workgroup through LDS; k_grouped_dot runs many problems in one
+//    persistent launch.  AXPY form (A contiguous along M): k_skinny_axpy
+//    reads A in operand order and B packed once per call by k_pack_b.
+//  * K is split across waves (inside a workgroup, summed in LDS) and, when
+//    rows alone cannot fill 256 CUs, across workgroups, whose partial tiles
+//    are summed in a fixed order by a second kernel (deterministic, no
+//    atomics).
 #include <algorithm>
 #include <mutex>
 
