@@ -208,17 +208,18 @@ PRECISION = {"fp32": 0, "bf16x3": 1, "bf16x6": 2}
 
 
 KMEANS_SEPARATE_ROUNDING = 0x1
+KMEANS_KERNEL = {None: 0, "screened": 0x2, "direct": 0x4}
 
 
 def kmeans_assign(rows, ctrds, assign, num_rows, num_dims, num_clusts,
-                  separate_rounding=False, stream=None):
+                  separate_rounding=False, kernel=None, stream=None):
     """assign[r] = nearest centroid of row r (hmlKmeansAssign) on device
     buffers: rows num_rows x num_dims, ctrds num_clusts x num_dims, both
-    row-major float32; assign uint32."""
+    row-major float32; assign uint32.  kernel: None (by size), "screened"
+    or "direct" (same indices)."""
+    flags = (KMEANS_SEPARATE_ROUNDING if separate_rounding else 0) | KMEANS_KERNEL[kernel]
     check(lib.hmlKmeansAssign(_ptr(assign), _ptr(rows), num_rows, num_dims,
-                              _ptr(ctrds), num_clusts,
-                              KMEANS_SEPARATE_ROUNDING if separate_rounding else 0,
-                              stream))
+                              _ptr(ctrds), num_clusts, flags, stream))
 
 
 def sgemm_dense(M, N, K, A, B, C, alpha=1.0, beta=0.0, precision="fp32",

@@ -163,17 +163,24 @@ void grouped_class_shape(int cls, int *rt, int *ct);
 int grouped_class_blocks(int cls);
 hipError_t launch_grouped(int cls, const GroupProblem *probs, const GroupTile *tiles,
                           int64_t ntiles, int nblocks, hipStream_t st);
-// k-means assignment (hml_k_kmeans.hip).  kKmWaves waves per workgroup,
-// each on its own 16-row tile; NI = dims / 16 rounded up, NT centroid tiles
-// of 16 per screening pass, KC centroids per LDS chunk.
-constexpr int kKmWaves = 4;
-constexpr size_t kKmLdsBytes = 65536;
+// k-means assignment (hml_k_kmeans.hip).  Screened form: kKmWaves waves
+// per workgroup, each on its own 16-row tile; NI = dims / 16 rounded up, NT
+// centroid tiles of 16 per screening pass.  Direct form (up to 8 dims, or
+// clusters x dims < 1024 up to 32 dims): kKmDirectWaves waves, each on 64
+// rows, one per lane; NI = dims.  KC centroids per LDS chunk; tiles = wave tiles of rows.
+constexpr int kKmWaves = 8;
+constexpr size_t kKmLdsBytes = 163840;      // all of a CU's LDS
+constexpr int kKmDirectWaves = 4;
+constexpr size_t kKmDirectLdsBytes = 65536; // centroid chunk of the direct form
+constexpr int kKmDirectMaxDims = 32;
 struct KmeansPlan {
+  bool direct = false;
   int ni = 1, nt = 1, KC = 0;
   size_t lds = 0;
   int64_t tiles = 0;
 };
-KmeansPlan plan_kmeans(int64_t nrows, int D, int K);
+// mode: 0 by size, 1 screened, 2 direct
+KmeansPlan plan_kmeans(int64_t nrows, int D, int K, int mode);
 hipError_t launch_kmeans_assign(const KmeansPlan &pl, uint32_t *assign, const float *rows,
                                 int64_t nrows, int D, const float *ctrds, int K, bool fused,
                                 hipStream_t st);

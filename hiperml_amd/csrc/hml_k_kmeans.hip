@@ -36,6 +36,8 @@
 // with a strict comparison, so any partition gives the same answer.
 #include <float.h>
 
+#include <utility>
+
 #include "hml_internal.h"
 #include "hml_kdev.h"
 
@@ -49,7 +51,7 @@ struct KmParams {
   const float *ctrds;
   uint32_t *assign;
   int64_t nrows;
-  int64_t steps;  // row steps of kKmWaves tiles per workgroup (uniform)
+  int64_t steps;  // row steps of kKmWaves wave tiles per workgroup (uniform)
   int32_t D, K;
   int32_t KC;     // centroids per LDS chunk (multiple of 16 NT)
 };
@@ -66,23 +68,37 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // Exact distance of LDS row xr to LDS centroid cr over nd4 groups of 4
 // dims (dims past D are zero in both, and adding (+0)^2 to a nonnegative
-// sum changes nothing).
+// sum changes nothing).  The groups are read 4 at a time before their
+// arithmetic, so the LDS latency is paid once per 16 dims.
 template <bool FUSED>
+__device__ __forceinline__ float exact_step(float dist, float x, float c) {
+  const float delta = x - c;
+  if (FUSED) return __builtin_fmaf(delta, delta, dist);
+  {
+#pragma clang fp contract(off)
+    const float sq = delta * delta;
+    return dist + sq;
+  }
+}
+
+template <bool FUSED, int NI>
 __device__ __forceinline__ float exact_dist(const float *xr, const float *cr, int nd4) {
   float dist = 0.f;
-  for (int d = 0; d < nd4; ++d) {
-    const f4 xv = *reinterpret_cast<const f4 *>(xr + 4 * d);
-    const f4 cv = *reinterpret_cast<const f4 *>(cr + 4 * d);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const float delta = xv[e] - cv[e];
-      if (FUSED) {
-        dist = __builtin_fmaf(delta, delta, dist);
-      } else {
-#pragma clang fp contract(off)
-        const float sq = delta * delta;
-        dist = dist + sq;
+  for (int g = 0; g < NI; ++g) {
+    if (4 * g < nd4) {
+      f4 xv[4], cv[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        xv[j] = *reinterpret_cast<const f4 *>(xr + 16 * g + 4 * j);
+        cv[j] = *reinterpret_cast<const f4 *>(cr + 16 * g + 4 * j);
       }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (4 * g + j < nd4) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) dist = exact_step<FUSED>(dist, xv[j][e], cv[j][e]);
+        }
     }
   }
   return dist;
@@ -95,8 +111,9 @@ __global__ __launch_bounds__(64 * kKmWaves) void k_kmeans_assign(KmParams p) {
   float *lds = reinterpret_cast<float *>(km_lds4);
   float *cs = lds;                   // KC x DP centroids of the current chunk
   float *cn = cs + p.KC * DP;        // their squared norms
+  float *cr = cn + p.KC;             // and norms
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  float *xs = cn + p.KC + w * 16 * DP;  // this wave's 16 rows
+  float *xs = cr + p.KC + w * 16 * DP;  // this wave's 16 rows
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, r = lane & 15;
   const int D = p.D, K = p.K;
@@ -115,10 +132,42 @@ __global__ __launch_bounds__(64 * kKmWaves) void k_kmeans_assign(KmParams p) {
       float s = 0.f;
       for (int d = 0; d < D; ++d) s = __builtin_fmaf(cs[k * DP + d], cs[k * DP + d], s);
       cn[k] = s;
+      cr[k] = __builtin_sqrtf(s);
     }
     __syncthreads();
   };
   if (nchunks == 1) stage(0);
+
+  // rows of step st's tile (zeros for a tile past the end); the next step's
+  // rows are loaded while this step's are screened
+  auto load_rows = [&](int64_t st, f4 (&dst)[NI]) {
+    const int64_t row0 = ((st * gridDim.x + blockIdx.x) * kKmWaves + w) * 16;
+    if (row0 >= p.nrows) {
+#pragma unroll
+      for (int i = 0; i < NI; ++i) dst[i] = f4{0.f, 0.f, 0.f, 0.f};
+      return;
+    }
+    const int64_t left = (p.nrows - row0) * D * 4;
+    const __amdgpu_buffer_rsrc_t rs =
+        make_rsrc(p.rows + row0 * D, left < kRsrcMax ? (int32_t)left : kRsrcMax);
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int d0 = 16 * i + 4 * q;
+      const bool rin = row0 + r < p.nrows;
+      const uint32_t o = (uint32_t)((r * D + d0) * 4);
+      if (d0 + 4 <= D) {
+        dst[i] = buf_load4<kBufNT>(rs, rin ? o : kOOB, 0);
+      } else {
+        // a group that crosses the end of the row, element by element
+        // (its tail belongs to the next row, or lies past the array)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          dst[i][e] = buf_load1(rs, rin && d0 + e < D ? o + 4 * e : kOOB, 0);
+      }
+    }
+  };
+  f4 xnext[NI];
+  load_rows(0, xnext);
 
   for (int64_t st = 0; st < p.steps; ++st) {
     const int64_t tile = (st * gridDim.x + blockIdx.x) * kKmWaves + w;
@@ -127,26 +176,11 @@ __global__ __launch_bounds__(64 * kKmWaves) void k_kmeans_assign(KmParams p) {
     if (nchunks == 1 && !live) break;  // later steps only go further
 
     f4 xv[NI];
-    float xx = 0.f;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) xv[i] = xnext[i];
+    if (st + 1 < p.steps) load_rows(st + 1, xnext);
+    float xx = 0.f, xr = 0.f;
     if (live) {
-      const int64_t left = (p.nrows - row0) * D * 4;
-      const __amdgpu_buffer_rsrc_t rs =
-          make_rsrc(p.rows + row0 * D, left < kRsrcMax ? (int32_t)left : kRsrcMax);
-#pragma unroll
-      for (int i = 0; i < NI; ++i) {
-        const int d0 = 16 * i + 4 * q;
-        const bool rin = row0 + r < p.nrows;
-        const uint32_t o = (uint32_t)((r * D + d0) * 4);
-        if (d0 + 4 <= D) {
-          xv[i] = buf_load4<kBufNT>(rs, rin ? o : kOOB, 0);
-        } else {
-          // a group that crosses the end of the row, element by element
-          // (its tail belongs to the next row, or lies past the array)
-#pragma unroll
-          for (int e = 0; e < 4; ++e)
-            xv[i][e] = buf_load1(rs, rin && d0 + e < D ? o + 4 * e : kOOB, 0);
-        }
-      }
       wave_lds_sync();  // the previous tile's refinement reads are done
 #pragma unroll
       for (int i = 0; i < NI; ++i) {
@@ -156,6 +190,7 @@ __global__ __launch_bounds__(64 * kKmWaves) void k_kmeans_assign(KmParams p) {
       }
       xx += __shfl_xor(xx, 16);
       xx += __shfl_xor(xx, 32);
+      xr = __builtin_sqrtf(xx);
       wave_lds_sync();
     }
 
@@ -179,15 +214,18 @@ __global__ __launch_bounds__(64 * kKmWaves) void k_kmeans_assign(KmParams p) {
           }
         }
         // screen: threshold over the pass, then candidates
+        // s = |c|^2 - 2 x.c, m = tau ((|x| + |c|)^2 + |s|)
         float T = __builtin_inff();
-        f4 nrm[NT];
+        f4 nrm[NT], nrt[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
           nrm[t] = *reinterpret_cast<const f4 *>(cn + pb + 16 * t + 4 * q);
+          nrt[t] = *reinterpret_cast<const f4 *>(cr + pb + 16 * t + 4 * q);
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            const float s = nrm[t][i] - 2.f * acc[t][i];
-            const float m = tau * (xx + nrm[t][i] + 2.f * __builtin_sqrtf(xx * nrm[t][i]) + __builtin_fabsf(s));
+            const float s = __builtin_fmaf(-2.f, acc[t][i], nrm[t][i]);
+            const float sum = xr + nrt[t][i];
+            const float m = tau * __builtin_fmaf(sum, sum, __builtin_fabsf(s));
             if (pb + 16 * t + 4 * q + i < kc) T = __builtin_fminf(T, s + m);
           }
         }
@@ -198,8 +236,9 @@ __global__ __launch_bounds__(64 * kKmWaves) void k_kmeans_assign(KmParams p) {
         for (int t = 0; t < NT; ++t)
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            const float s = nrm[t][i] - 2.f * acc[t][i];
-            const float m = tau * (xx + nrm[t][i] + 2.f * __builtin_sqrtf(xx * nrm[t][i]) + __builtin_fabsf(s));
+            const float s = __builtin_fmaf(-2.f, acc[t][i], nrm[t][i]);
+            const float sum = xr + nrt[t][i];
+            const float m = tau * __builtin_fmaf(sum, sum, __builtin_fabsf(s));
             if (pb + 16 * t + 4 * q + i < kc && !(s - m > T)) cand |= 1u << (4 * t + i);
           }
         // exact refinement, ascending k within the lane
@@ -209,7 +248,7 @@ __global__ __launch_bounds__(64 * kKmWaves) void k_kmeans_assign(KmParams p) {
           const int b = __builtin_ctz(cand);
           cand &= cand - 1;
           const int kl = pb + 16 * (b >> 2) + 4 * q + (b & 3);
-          const float dist = exact_dist<FUSED>(xs + r * DP, cs + kl * DP, nd4);
+          const float dist = exact_dist<FUSED, NI>(xs + r * DP, cs + kl * DP, nd4);
           if (dist < bd) {
             bd = dist;
             bk = (uint32_t)(c * p.KC + kl);
@@ -236,6 +275,127 @@ __global__ __launch_bounds__(64 * kKmWaves) void k_kmeans_assign(KmParams p) {
 }
 
 typedef void (*KmKernel)(KmParams);
+
+// ---------------------------------------------------------------------------
+// Direct form, for narrow rows: each lane takes one row (64 rows per wave)
+// and runs the reference's loop itself -- per centroid ascending, per dim
+// ascending, one subtraction and one fused (or separately rounded)
+// multiply-add -- with the centroids staged in LDS (row stride 4 ceil(D/4),
+// zero padded) and read by all lanes at once (broadcast 16-byte reads).
+// The wave's 64 x D block of rows is contiguous: it is loaded with
+// coalesced 16-byte loads (the next step's while this step computes),
+// split into rows in the wave's LDS area (odd row stride: conflict-free)
+// and read back one row per lane.  When the centroids exceed one LDS chunk
+// the workgroup's waves step through the chunks in lockstep.
+// ---------------------------------------------------------------------------
+template <int D, bool FUSED>
+__global__ __launch_bounds__(64 * kKmDirectWaves) void k_kmeans_direct(KmParams p) {
+  constexpr int DS = (D + 3) / 4 * 4;   // staged floats per centroid
+  constexpr int XS = D | 1;             // LDS row stride of the row block
+  constexpr int NP = (16 * D + 63) / 64;  // 16-byte pieces per lane per block
+  extern __shared__ f4 kmd_lds4[];
+  float *cs = reinterpret_cast<float *>(kmd_lds4);  // KC x DS
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  float *xs = cs + p.KC * DS + w * 64 * XS;  // this wave's 64 rows
+  const int K = p.K;
+  const int nchunks = (K + p.KC - 1) / p.KC;
+  auto stage = [&](int c0) {
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < p.KC * DS; idx += blockDim.x) {
+      const int k = idx / DS, d = idx - k * DS;
+      cs[idx] = (c0 + k < K && d < D) ? p.ctrds[(int64_t)(c0 + k) * D + d] : 0.f;
+    }
+    __syncthreads();
+  };
+  if (nchunks == 1) stage(0);
+  auto row0_of = [&](int64_t st) {
+    return ((st * gridDim.x + blockIdx.x) * kKmDirectWaves + w) * 64;
+  };
+  // piece f = lane + 64 i of the block: floats 4f .. 4f+3 (past the end of
+  // the array: zero; a piece that crosses it element by element)
+  auto load_block = [&](int64_t st, f4 (&dst)[NP]) {
+    const int64_t row0 = row0_of(st);
+    const int64_t left = row0 < p.nrows ? (p.nrows - row0) * D * 4 : 0;
+    const __amdgpu_buffer_rsrc_t rs =
+        make_rsrc(p.rows + (row0 < p.nrows ? row0 : 0) * D, left < kRsrcMax ? (int32_t)left : kRsrcMax);
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      const int64_t o = 16 * (int64_t)(lane + 64 * i);
+      if (lane + 64 * i >= 16 * D || o >= left) {
+        dst[i] = f4{0.f, 0.f, 0.f, 0.f};
+      } else if (o + 16 <= left) {
+        dst[i] = buf_load4<kBufNT>(rs, (uint32_t)o, 0);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) dst[i][e] = buf_load1(rs, o + 4 * e < left ? (uint32_t)(o + 4 * e) : kOOB, 0);
+      }
+    }
+  };
+  f4 pnext[NP];
+  load_block(0, pnext);
+  for (int64_t st = 0; st < p.steps; ++st) {
+    const int64_t row0 = row0_of(st);
+    const bool live = row0 < p.nrows;  // wave-uniform
+    if (nchunks == 1 && !live) break;
+    float x[DS];
+    if (live) {
+      // split the block into rows: element 4f + e is row (4f+e)/D, dim (4f+e)%D
+#pragma unroll
+      for (int i = 0; i < NP; ++i) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int idx = 4 * (lane + 64 * i) + e;
+          if (idx < 64 * D) xs[(idx / D) * XS + idx % D] = pnext[i][e];
+        }
+      }
+      wave_lds_sync();
+#pragma unroll
+      for (int d = 0; d < DS; ++d) x[d] = d < D ? xs[lane * XS + d] : 0.f;
+      wave_lds_sync();  // the next step's split may overwrite the area
+    }
+    if (st + 1 < p.steps) load_block(st + 1, pnext);
+    const bool rin = row0 + lane < p.nrows;
+    float best = FLT_MAX;
+    uint32_t bk = kNone;
+    for (int c = 0; c < nchunks; ++c) {
+      if (nchunks > 1) stage(c * p.KC);
+      if (!live) continue;
+      const int kc = min(p.KC, K - c * p.KC);
+      for (int k = 0; k < kc; ++k) {
+        const float *cr = cs + k * DS;
+        float dist = 0.f;
+#pragma unroll
+        for (int j = 0; j < DS / 4; ++j) {
+          const f4 cv = *reinterpret_cast<const f4 *>(cr + 4 * j);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) dist = exact_step<FUSED>(dist, x[4 * j + e], cv[e]);
+        }
+        if (dist < best) {
+          best = dist;
+          bk = (uint32_t)(c * p.KC + k);
+        }
+      }
+    }
+    if (live && rin) p.assign[row0 + lane] = bk == kNone ? (uint32_t)K : bk;
+  }
+}
+
+template <int D>
+KmKernel kmd_pick(bool fused) {
+  return fused ? k_kmeans_direct<D, true> : k_kmeans_direct<D, false>;
+}
+
+template <int... Ds>
+KmKernel kmd_find(int d, bool fused, std::integer_sequence<int, Ds...>) {
+  KmKernel out = nullptr;
+  ((d == Ds + 1 ? (out = kmd_pick<Ds + 1>(fused), 0) : 0), ...);
+  return out;
+}
+
+KmKernel kmd_kernel(int d, bool fused) {
+  return kmd_find(d, fused, std::make_integer_sequence<int, kKmDirectMaxDims>{});
+}
 
 template <int NI, int NT>
 KmKernel km_pick(bool fused) {
@@ -271,21 +431,40 @@ int cu_count() {
 
 }  // namespace
 
-KmeansPlan plan_kmeans(int64_t nrows, int D, int K) {
+KmeansPlan plan_kmeans(int64_t nrows, int D, int K, int mode) {
   KmeansPlan pl;
+  // measured on MI355X (2^24 rows, dims 4..32 x clusters 4..1024): the
+  // direct kernel wins up to 8 dims at any cluster count and below
+  // clusters x dims = 1024 up to 32 dims (by up to 9x at few clusters);
+  // the screened kernel wins above
+  pl.direct = (mode == 2 && D <= kKmDirectMaxDims) ||
+              (mode == 0 && D <= kKmDirectMaxDims && (D <= 8 || (int64_t)K * D < 1024));
+  if (pl.direct) {
+    // one row per lane; centroids in chunks of at most kKmDirectLdsBytes,
+    // plus each wave's 64-row area
+    pl.ni = D;
+    const int64_t ds = (D + 3) / 4 * 4;
+    const int64_t rows_f = (int64_t)kKmDirectWaves * 64 * (D | 1);
+    int64_t kc = (int64_t)(kKmDirectLdsBytes / 4) / ds;
+    if (kc > K) kc = K;
+    pl.KC = (int)kc;
+    pl.lds = (size_t)(kc * ds + rows_f) * 4;
+    pl.tiles = (nrows + 63) / 64;  // 64-row wave blocks
+    return pl;
+  }
   pl.ni = (D + 15) / 16;
   pl.nt = K <= 16 ? 1 : K <= 32 ? 2 : K <= 64 ? 4 : 8;
   const int dp = 16 * pl.ni + 4;
   const int pass = 16 * pl.nt;
-  // LDS: centroid chunk (KC x dp) + norms (KC) + 16 rows per wave, at most
-  // kKmLdsBytes.
+  // LDS: centroid chunk (KC x dp) + squared norms and norms (2 KC) + 16 rows
+  // per wave, at most kKmLdsBytes.
   const int64_t rows_f = (int64_t)kKmWaves * 16 * dp;
   const int64_t room = kKmLdsBytes / 4 - rows_f;
-  int64_t kc = room / (dp + 1) / pass * pass;
+  int64_t kc = room / (dp + 2) / pass * pass;
   const int64_t kpad = ((int64_t)K + pass - 1) / pass * pass;
   if (kc > kpad) kc = kpad;
   pl.KC = (int)kc;
-  pl.lds = (size_t)(kc * (dp + 1) + rows_f) * 4;
+  pl.lds = (size_t)(kc * (dp + 2) + rows_f) * 4;
   pl.tiles = (nrows + 15) / 16;
   return pl;
 }
@@ -293,14 +472,23 @@ KmeansPlan plan_kmeans(int64_t nrows, int D, int K) {
 hipError_t launch_kmeans_assign(const KmeansPlan &pl, uint32_t *assign, const float *rows,
                                 int64_t nrows, int D, const float *ctrds, int K, bool fused,
                                 hipStream_t st) {
-  if (pl.KC < 16 * pl.nt || nrows <= 0) return nrows <= 0 ? hipSuccess : hipErrorInvalidValue;
-  KmKernel fn = km_kernel(pl.ni, pl.nt, fused);
+  if (nrows <= 0) return hipSuccess;
+  if (pl.direct ? pl.KC < 1 : pl.KC < 16 * pl.nt) return hipErrorInvalidValue;
+  KmKernel fn = pl.direct ? kmd_kernel(pl.ni, fused) : km_kernel(pl.ni, pl.nt, fused);
+  if (!fn) return hipErrorInvalidValue;
+  const int waves = pl.direct ? kKmDirectWaves : kKmWaves;
+  hipError_t e;
+  if (pl.lds > 65536) {
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds);
+    if (e != hipSuccess) return e;
+  }
   int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(fn),
-                                                              64 * kKmWaves, pl.lds);
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(fn),
+                                                   64 * waves, pl.lds);
   if (e != hipSuccess) return e;
   if (per_cu < 1) per_cu = 1;
-  const int64_t wgs_needed = (pl.tiles + kKmWaves - 1) / kKmWaves;
+  const int64_t wgs_needed = (pl.tiles + waves - 1) / waves;
   int64_t grid = (int64_t)per_cu * cu_count();
   if (grid > wgs_needed) grid = wgs_needed;
   KmParams p;
@@ -308,11 +496,11 @@ hipError_t launch_kmeans_assign(const KmeansPlan &pl, uint32_t *assign, const fl
   p.ctrds = ctrds;
   p.assign = assign;
   p.nrows = nrows;
-  p.steps = (pl.tiles + grid * kKmWaves - 1) / (grid * kKmWaves);
+  p.steps = (pl.tiles + grid * waves - 1) / (grid * waves);
   p.D = D;
   p.K = K;
   p.KC = pl.KC;
-  hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(64 * kKmWaves), pl.lds, st, p);
+  hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(64 * waves), pl.lds, st, p);
   return hipGetLastError();
 }
 

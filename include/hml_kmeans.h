@@ -31,6 +31,14 @@ extern "C" {
  *              path runs on baseline x86-64. */
 #define HML_KMEANS_SEPARATE_ROUNDING 0x1u
 
+/* Kernel choice; the indices are the same either way.  By default up to 8
+ * dims, or numClusts x numDims < 1024 up to 32 dims, run the direct kernel
+ * (one row per lane, the reference loop), larger problems the screened
+ * kernel (x.c on the matrix cores, exact refinement of the candidates).  These force one;
+ * the direct kernel takes at most 32 dims (HML_STATUS_UNSUPPORTED above). */
+#define HML_KMEANS_KERNEL_SCREENED 0x2u
+#define HML_KMEANS_KERNEL_DIRECT   0x4u
+
 /* assign[r] = the nearest centroid of row r (r < numRows).
  *   rows   numRows x numDims floats, row-major (the reference's pRows)
  *   ctrds  numClusts x numDims floats, row-major (pCtrds)

@@ -72,7 +72,8 @@ def test_argument_errors_without_gpu():
     assert km(1, 1, 10, 65, 1, 4, 0, None) == 1
     assert b"1..64" in hml.lib.hmlLastError()
     assert km(1, 1, 10, 8, 1, 0, 0, None) == 1
-    assert km(1, 1, 10, 8, 1, 4, 2, None) == 1
+    assert km(1, 1, 10, 8, 1, 4, 8, None) == 1         # unknown flag
+    assert km(1, 1, 10, 8, 1, 4, 6, None) == 1         # both kernels forced
     assert km(None, 1, 10, 8, 1, 4, 0, None) == 1
     assert km(None, None, 0, 8, None, 4, 1, None) == 0
 

@@ -13,14 +13,23 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 
-def run(hml, rows, ctrds, separate=False, offset=0):
+KERNELS = [None, "screened", "direct"]
+
+
+def kernels(D):
+    """Kernel choices to check for D dims (the direct one takes <= 32)."""
+    return KERNELS if D <= 32 else KERNELS[:2]
+
+
+def run(hml, rows, ctrds, separate=False, offset=0, kernel=None):
     n, D = rows.shape
     K = ctrds.shape[0]
     flat = np.concatenate([np.full(offset, 7, np.float32), rows.ravel()]).astype(np.float32)
     dr = hml.DeviceBuffer.from_numpy(flat)
     dc = hml.DeviceBuffer.from_numpy(np.ascontiguousarray(ctrds, np.float32))
     da = hml.DeviceBuffer(4 * max(n, 1))
-    hml.kmeans_assign(dr.ptr + 4 * offset, dc, da, n, D, K, separate_rounding=separate)
+    hml.kmeans_assign(dr.ptr + 4 * offset, dc, da, n, D, K, separate_rounding=separate,
+                      kernel=kernel)
     hml.synchronize()
     return da.to_numpy((n,), np.uint32)
 
@@ -47,21 +56,26 @@ def test_matches_oracle(hml, D, K):
     for make in (clustered, near_ties):
         rows, ctrds = make(rng, n, D, K)
         for sep in (False, True):
-            got = run(hml, rows, ctrds, sep)
-            np.testing.assert_array_equal(got, oracle.kmeans_assign(rows, ctrds, fused=not sep),
-                                          err_msg="%s sep=%s" % (make.__name__, sep))
+            want = oracle.kmeans_assign(rows, ctrds, fused=not sep)
+            for kern in kernels(D):
+                got = run(hml, rows, ctrds, sep, kernel=kern)
+                np.testing.assert_array_equal(got, want, err_msg="%s sep=%s kernel=%s" % (
+                    make.__name__, sep, kern))
 
 
-@pytest.mark.parametrize("D,K", [(64, 129), (64, 300), (16, 1000), (8, 4097), (2, 70)])
+@pytest.mark.parametrize("D,K", [(64, 129), (64, 300), (16, 1000), (8, 4097), (2, 70), (64, 1000), (3, 9000)])
 def test_many_centroids(hml, D, K):
-    """More centroids than one LDS chunk holds (several chunks per row step)
-    and several screening passes."""
+    """More centroids than one LDS chunk holds (several chunks per row step:
+    above 256 x 64 dims for the direct kernel, 128 x 64 for the screened
+    one) and several screening passes."""
     rng = np.random.default_rng(K)
     print("case D=%d K=%d" % (D, K), flush=True)
     for make in (clustered, near_ties):
         rows, ctrds = make(rng, 2000, D, K)
-        got = run(hml, rows, ctrds)
-        np.testing.assert_array_equal(got, oracle.kmeans_assign(rows, ctrds), err_msg=make.__name__)
+        want = oracle.kmeans_assign(rows, ctrds)
+        for kern in kernels(D):
+            got = run(hml, rows, ctrds, kernel=kern)
+            np.testing.assert_array_equal(got, want, err_msg="%s kernel=%s" % (make.__name__, kern))
 
 
 def test_fused_vs_separate_flip(hml):
@@ -70,8 +84,9 @@ def test_fused_vs_separate_flip(hml):
     from test_oracle import fused_flip_case
     rows, ctrds = fused_flip_case()
     rows = np.repeat(rows, 37, 0)
-    assert (run(hml, rows, ctrds, separate=False) == 1).all()
-    assert (run(hml, rows, ctrds, separate=True) == 0).all()
+    for kern in KERNELS:
+        assert (run(hml, rows, ctrds, separate=False, kernel=kern) == 1).all()
+        assert (run(hml, rows, ctrds, separate=True, kernel=kern) == 0).all()
 
 
 def test_edge_rows(hml):
@@ -91,16 +106,19 @@ def test_edge_rows(hml):
     assert want[7] == K and want[8] == K and want[9] == K
     assert not np.isin(want, [30, 31]).any()
     for off in (0, 1):
-        np.testing.assert_array_equal(run(hml, rows, ctrds, offset=off), want)
+        for kern in KERNELS:
+            np.testing.assert_array_equal(run(hml, rows, ctrds, offset=off, kernel=kern), want)
 
 
 def test_sizes(hml):
     """Row counts around the 16-row tile and the 4-tile workgroup step, one
     cluster, and a large problem checked through a sample of rows."""
     rng = np.random.default_rng(5)
-    for n in (1, 15, 16, 17, 63, 64, 65, 1000):
+    for n in (1, 15, 16, 17, 63, 64, 65, 255, 256, 257, 1000):
         rows, ctrds = clustered(rng, n, 10, 7)
-        np.testing.assert_array_equal(run(hml, rows, ctrds), oracle.kmeans_assign(rows, ctrds))
+        for kern in KERNELS:
+            np.testing.assert_array_equal(run(hml, rows, ctrds, kernel=kern),
+                                          oracle.kmeans_assign(rows, ctrds))
     rows, ctrds = clustered(rng, 100, 5, 1)
     assert (run(hml, rows, ctrds) == 0).all()
     n, D, K = 1 << 22, 32, 64
@@ -115,3 +133,11 @@ def test_sizes(hml):
     d_own = ((rows - ctrds[got]) ** 2).sum(1)
     d_oth = ((rows - ctrds[other]) ** 2).sum(1)
     assert (d_own <= d_oth * (1 + 1e-5) + 1e-30).all()
+
+
+def test_direct_kernel_dim_limit(hml):
+    """Forcing the direct kernel above 32 dims is rejected, not rerouted."""
+    rows = np.zeros((10, 33), np.float32)
+    ctrds = np.zeros((3, 33), np.float32)
+    with pytest.raises(hml.HmlError):
+        run(hml, rows, ctrds, kernel="direct")
