@@ -230,9 +230,9 @@ void heuristic_sgemm(const GemmKey &k, hmlSgemmVariant *v) {
     tiles = ceil_div(k.M, 64) * ceil_div(nct, v->ct);
   }
   // About one resident round of waves: for the DOT kernel the number its
-  // register and LDS budget lets the device hold at once (3072 for 2 x 1
-  // tiles on MI355X: 768 workgroups of 4 waves), so the whole split runs
-  // concurrently; each K chunk at least 256 long.
+  // register and LDS budget lets the device hold at once (from the
+  // occupancy calculator), so the whole split runs concurrently; each K
+  // chunk at least 256 long.
   int64_t target = 4096;
   if (k.form == kFormDot) {
     const int rw = skinny_dot_resident_waves(v->rt, v->ct);

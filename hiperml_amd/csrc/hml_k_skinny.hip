@@ -163,32 +163,31 @@ __device__ __forceinline__ void dot_tile_lds(const float *__restrict__ A, int64_
 #pragma unroll
     for (int c = 0; c < CT; ++c) acc[t][c] = f4{0.f, 0.f, 0.f, 0.f};
   f4 a[PS][NQ];
-  f4 b[PS][CT][2][2];  // [stage set][column tile][k-step u][half v]
+  f4 b[CT][2][2];  // one stage of B: [column tile][k-step u][half v]
   auto loadA = [&](int j, int64_t s) {
     const __amdgpu_buffer_rsrc_t rs = s < nst ? ra : rnull;
     const int32_t so = (int32_t)(s * kDotStageK * 4);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) a[j][q] = buf_load4<kBufNT>(rs, aoff[q], so);
   };
-  auto loadB = [&](int j, int64_t s) {
+  // k-step u (32 k) of stage s's B
+  auto loadBu = [&](int u, int64_t s) {
     const __amdgpu_buffer_rsrc_t rs = s < nst ? rbs : rnull;
 #pragma unroll
     for (int c = 0; c < CT; ++c)
 #pragma unroll
-      for (int u = 0; u < 2; ++u)
+      for (int v = 0; v < 2; ++v) {
+        const int64_t ko = s * kDotStageK + 32 * u + 16 * v;
+        f4 x;
+        if (BVEC) {
+          x = buf_load4<0>(rs, boff[c], (int32_t)(ko * 4));
+        } else {
 #pragma unroll
-        for (int v = 0; v < 2; ++v) {
-          const int64_t ko = s * kDotStageK + 32 * u + 16 * v;
-          f4 x;
-          if (BVEC) {
-            x = buf_load4<0>(rs, boff[c], (int32_t)(ko * 4));
-          } else {
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj)
-              x[jj] = buf_load1(rs, boff[c], (int32_t)((ko + jj) * sBk * 4));
-          }
-          b[j][c][u][v] = x;
+          for (int jj = 0; jj < 4; ++jj)
+            x[jj] = buf_load1(rs, boff[c], (int32_t)((ko + jj) * sBk * 4));
         }
+        b[c][u][v] = x;
+      }
   };
   auto put = [&](int j) {
 #pragma unroll
@@ -198,7 +197,9 @@ __device__ __forceinline__ void dot_tile_lds(const float *__restrict__ A, int64_
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
-  auto compute = [&](int j) {
+  // MFMAs of the stage in the tile and b; each k-step's B registers are
+  // refilled with stage s_next's as soon as its MFMAs are issued
+  auto compute = [&](int64_t s_next) {
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       f4 x[RT][2];
@@ -215,7 +216,8 @@ __device__ __forceinline__ void dot_tile_lds(const float *__restrict__ A, int64_
           for (int t = 0; t < RT; ++t)
 #pragma unroll
             for (int c = 0; c < CT; ++c)
-              acc[t][c] = mfma4(x[t][v][jj], b[j][c][u][v][jj], acc[t][c]);
+              acc[t][c] = mfma4(x[t][v][jj], b[c][u][v][jj], acc[t][c]);
+      loadBu(u, s_next);
     }
     // the next put() may overwrite the tile only after these reads
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -223,23 +225,23 @@ __device__ __forceinline__ void dot_tile_lds(const float *__restrict__ A, int64_
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
 
-  // Full stages, PS at a time; stage t uses register set t % PS.  A stage's
-  // A registers are refilled right after they reach LDS, its B registers
-  // after the MFMAs have read them.
+  // Full stages, PS at a time; stage t's A uses register set t % PS and is
+  // loaded PS stages ahead (after the previous stage's B, so that waiting
+  // for B never waits for later A); B is loaded one stage ahead.
 #pragma unroll
-  for (int j = 0; j < PS; ++j) {
-    loadA(j, j);
-    loadB(j, j);
-    __builtin_amdgcn_sched_barrier(0);  // issue the stages in order
-  }
+  for (int j = 0; j < PS; ++j) loadA(j, j);
+  loadBu(0, 0);
+  loadBu(1, 0);
+  __builtin_amdgcn_sched_barrier(0);
   for (int64_t s = 0; s < nst; s += PS) {
 #pragma unroll
     for (int j = 0; j < PS; ++j) {
       const bool live = s + j < nst;
-      if (live) put(j);
+      if (live) {
+        put(j);
+        compute(s + j + 1);
+      }
       loadA(j, s + PS + j);
-      if (live) compute(j);
-      loadB(j, s + PS + j);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -273,10 +275,10 @@ __device__ __forceinline__ void dot_tile_lds(const float *__restrict__ A, int64_
                                 : boff[c] + (uint32_t)(ko * sBk * 4);
             x[jj] = buf_load1(rbs, vo, 0);
           }
-          b[0][c][u][v] = x;
+          b[c][u][v] = x;
         }
     put(0);
-    compute(0);
+    compute(nst + 1);  // nothing left to load: the refills read zeros, no traffic
   }
 
 }
