@@ -100,13 +100,13 @@ void heuristic_sgemv(const GemvKey &k, hmlSgemvVariant *v) {
     v->split = (int)std::max<int64_t>(1, std::min<int64_t>(split, 1024));
   } else {
     // Column chunks of at most 128 floats (32 lanes); the rows split so
-    // that about 1024 workgroups of 4 waves run, each split at least 64
-    // rows long, at most 512 splits.
+    // that about 512 workgroups of 4 waves (2048 waves) run, each split at
+    // least 64 rows long, at most 512 splits.
     v->lpr = std::min(32, lpr_for(k.outputs));
     v->rb = 1;
     v->wpb = 4;
     const int64_t ncb = ceil_div(k.outputs, 4 * v->lpr);
-    int64_t split = ceil_div(1024, ncb);
+    int64_t split = ceil_div(512, ncb);
     split = std::min<int64_t>(split, std::max<int64_t>(1, k.reduced / 64));
     v->split = (int)std::max<int64_t>(1, std::min<int64_t>(split, 512));
   }

@@ -58,3 +58,34 @@ def test_dot_variant_limits():
         if v.family != 1:
             continue
         assert v.ct <= 4 and v.rt * v.ct <= 8 and v.wpb in (1, 2, 4, 8)
+
+
+@pytest.mark.parametrize("M,N,K,ks", [(8192, 64, 8192, 8), (1000, 100, 129, 3), (300, 40, 64, 1)])
+def test_dotb_workspace(M, N, K, ks):
+    # shared-B DOT (family 7): every K chunk is a workgroup, so kg = chunks
+    kchunk = math.ceil(math.ceil(K / ks) / 64) * 64
+    kg = math.ceil(K / kchunk)
+    want = 0 if kg == 1 else (kg * M * N * 4 + 255) // 256 * 256
+    assert ws(M, N, K, (7, 2, 4, ks, 4)) == want
+
+
+def test_heuristic_dense_gate():
+    """The dense FP32 kernel (family 5, no K split) only when its 128 x 128
+    tiles fill the device and K is small or N exceeds 512; otherwise the
+    K-split skinny kernels (measured up to 2x faster there)."""
+    sel = lambda M, N, K: hml.sgemm_select(M, N, K)[0].family
+    assert sel(8192, 1024, 8192) == 5            # truly dense
+    assert sel(16384, 256, 256) == 5             # small K, tiles fill
+    assert sel(417, 512, 63616) != 5             # 16 tiles, long K
+    assert sel(32768, 132, 46598) != 5           # N <= 512, long K
+    assert sel(8192, 64, 8192) == 1              # skinny DOT
+    assert sel(8192, 1, 8192) == 3               # GEMV
+
+
+def test_gemv_heuristic_split_by_size():
+    """GEMV rows are split only for A >= 512 MB (about 64K waves), and
+    mid-size problems keep one pass (the split's reduction costs more)."""
+    v = lambda M, N: hml.sgemm_select(M, 1, N)[0]
+    big, mid = v(16384, 16384), v(8192, 8192)
+    assert big.family == 3 and big.ksplit > 1
+    assert mid.family == 3 and mid.ksplit == 1
