@@ -54,6 +54,8 @@ def main():
                     help="also run the split-precision dense suite (extra JSON line)")
     ap.add_argument("--grouped", action="store_true",
                     help="also run the grouped 512-problem suite (extra JSON line)")
+    ap.add_argument("--host", action="store_true",
+                    help="also time the host-pointer entry points (PCIe copies included)")
     ap.add_argument("--kmeans", action="store_true",
                     help="also run the k-means assignment suite (extra JSON line)")
     ap.add_argument("--only-extra", action="store_true",
@@ -86,6 +88,8 @@ def main():
             bench_grouped(hml)
         if args.kmeans:
             bench_kmeans(hml, cpu=not args.no_cpu)
+        if args.host:
+            bench_host(hml)
         return
     stream = hml.Stream()
     S = stream.ptr
@@ -244,6 +248,8 @@ def main():
         bench_grouped(hml)
     if args.kmeans:
         bench_kmeans(hml, cpu=not args.no_cpu)
+    if args.host:
+        bench_host(hml)
 
 
 def bandwidth_probe(hml, src, S, nbytes=256 << 20, reps=5):
@@ -434,6 +440,39 @@ def bench_kmeans(hml, reps=5, rows=1 << 24, cpu=True):
         R.free()
         A.free()
     print(json.dumps({"suite": "k-means assignment, %d rows" % rows, "results": out}))
+
+
+def bench_host(hml, reps=3):
+    """The reference's own contract: hmlSgemm / hmlSgemv on host arrays
+    (copy in, compute, copy out, synchronous), wall-clock per call, for
+    SGEMM 8192 x 16 x 8192 and SGEMV 16384 x 16384; the rate counts the
+    same algorithmic bytes as the device-resident sweep, so the gap to it
+    is the PCIe copies."""
+    rng = np.random.default_rng(0)
+    out = []
+    for kind, shape in (("sgemm", (8192, 16, 8192)), ("sgemv", (16384, 16384))):
+        if kind == "sgemm":
+            M, N, K = shape
+            A = rng.uniform(-1, 1, (M, K)).astype(np.float32)
+            B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+            C = np.zeros((M, N), np.float32)
+            fn = lambda: hml.sgemm(C, A, B, 1.0, 0.0)
+            nbytes = 4.0 * (M * K + K * N + M * N)
+        else:
+            M, N = shape
+            A = rng.uniform(-1, 1, (M, N)).astype(np.float32)
+            x = rng.uniform(-1, 1, N).astype(np.float32)
+            y = np.zeros(M, np.float32)
+            fn = lambda: hml.sgemv(y, A, x, 1.0, 0.0)
+            nbytes = 4.0 * (M * N + M + N)
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        dt = (time.perf_counter() - t0) / reps
+        out.append({"call": "%s %s (host pointers)" % (kind, "x".join(map(str, shape))),
+                    "ms": round(dt * 1e3, 3), "GB_s": round(nbytes / dt / 1e9, 1)})
+    print(json.dumps({"suite": "host-pointer entry points, PCIe copies included", "results": out}))
 
 
 CALIB_BYTES = 256 << 20
