@@ -89,3 +89,14 @@ def test_gemv_heuristic_split_by_size():
     big, mid = v(16384, 16384), v(8192, 8192)
     assert big.family == 3 and big.ksplit > 1
     assert mid.family == 3 and mid.ksplit == 1
+
+
+def test_heuristic_fp32_bound_split():
+    """N > 48: the K split stays inside the workgroup when the row tiles
+    fill a third of a resident round (no reduction pass), but a short, long-K
+    problem still splits across workgroups."""
+    sel = lambda M, N, K: hml.sgemm_select(M, N, K)[0]
+    big = sel(8192, 64, 8192)
+    assert big.family == 1 and big.ksplit <= big.wpb
+    thin = sel(1024, 64, 65536)
+    assert thin.family == 1 and thin.ksplit > thin.wpb
