@@ -116,18 +116,18 @@ GemmCanon dense_canon(hmlOrder order, hmlOp ta, hmlOp tb, int64_t M,
 namespace {
 
 // Resolved launch geometry of a skinny variant.
-//  AXPY form: K split into `ksplit` chunks over waves, partials reduced by a
-//    second kernel; B packed into Bp (nct_pad column tiles).
-//  DOT form (k_skinny_dot): K split into chunks of kchunk (a multiple of the
-//    64-k stage), kw consecutive chunks per workgroup summed in LDS, and
-//    kgroups workgroups along K whose partials are reduced when > 1; B is
-//    read in place.  variant.wpb bounds kw.
+//  Both forms: K split into chunks of kchunk (a multiple of the form's K
+//    step: 64 for DOT, 16 for AXPY), kw consecutive chunks per workgroup
+//    summed in LDS, and `ksplit` workgroups along K whose partials are
+//    reduced by a second kernel when > 1; B is read in place.  variant.wpb
+//    bounds kw (DOTB: waves take rows instead, kw = 1).  AXPY grids cover
+//    nct_pad column tiles (a multiple of the variant's CT).
 struct SkinnyPlan {
   int nct_pad;
   int64_t kchunk;
   int ksplit;   // partial sums to reduce (1: none)
-  int kw;       // DOT: waves per workgroup
-  size_t bp_floats, p_floats;
+  int kw;       // waves per workgroup along K
+  size_t p_floats;
 };
 
 SkinnyPlan plan_skinny(const GemmKey &k, const hmlSgemmVariant &v) {
@@ -135,38 +135,37 @@ SkinnyPlan plan_skinny(const GemmKey &k, const hmlSgemmVariant &v) {
   const int64_t nct = ceil_div(std::max<int64_t>(k.N, 1), 16);
   const int64_t Kc = std::max<int64_t>(k.K, 1);
   int64_t ks = std::max<int64_t>(1, v.ksplit);
-  if (k.form == kFormDot) {
-    pl.nct_pad = (int)nct;
-    int64_t kchunk = ceil_div(ceil_div(Kc, ks), kDotStageK) * kDotStageK;
-    const int64_t nchunks = ceil_div(Kc, kchunk);
-    pl.kchunk = kchunk;
-    if (v.family == HML_SGEMM_FAMILY_DOTB) {
-      // the waves of a workgroup take rows: every chunk is a workgroup
-      pl.kw = 1;
-      pl.ksplit = (int)nchunks;
-    } else {
-      pl.kw = (int)std::max<int64_t>(1, std::min<int64_t>(std::max(1, v.wpb), nchunks));
-      pl.ksplit = (int)ceil_div(nchunks, pl.kw);
-    }
-    pl.bp_floats = 0;
-  } else {
-    pl.nct_pad = (int)(ceil_div(nct, v.ct) * v.ct);
-    int64_t kchunk = ceil_div(Kc, ks);
-    kchunk = ceil_div(kchunk, kAxpyKStep) * kAxpyKStep;
-    pl.kchunk = kchunk;
-    pl.ksplit = (int)ceil_div(Kc, kchunk);
+  const int64_t step = k.form == kFormDot ? kDotStageK : kAxpyKStep;
+  const int64_t kchunk = ceil_div(ceil_div(Kc, ks), step) * step;
+  const int64_t nchunks = ceil_div(Kc, kchunk);
+  pl.kchunk = kchunk;
+  pl.nct_pad = k.form == kFormDot ? (int)nct : (int)(ceil_div(nct, v.ct) * v.ct);
+  if (v.family == HML_SGEMM_FAMILY_DOTB) {
+    // the waves of a workgroup take rows: every chunk is a workgroup
     pl.kw = 1;
-    pl.bp_floats = packed_b_floats(k.K, pl.nct_pad);
+    pl.ksplit = (int)nchunks;
+  } else {
+    pl.kw = (int)std::max<int64_t>(1, std::min<int64_t>(std::max(1, v.wpb), nchunks));
+    if (k.form == kFormAxpy)  // power-of-two workgroups
+      while (pl.kw & (pl.kw - 1)) --pl.kw;
+    pl.ksplit = (int)ceil_div(nchunks, pl.kw);
   }
   pl.p_floats = pl.ksplit > 1 ? (size_t)pl.ksplit * k.M * k.N : 0;
   return pl;
 }
 
-// Every in-range byte offset a DOT-form wave forms from its buffer
-// descriptors stays below 2^31 (k_skinny_dot's addressing contract).
-bool dot_offsets_fit(const GemmCanon &g, const hmlSgemmVariant &v,
-                     const SkinnyPlan &pl) {
+// Every in-range byte offset a skinny wave forms from its buffer
+// descriptors stays below 2^31 (the kernels' addressing contract).  The
+// AXPY kernel rebuilds its descriptors per 16-k step, so only one step's
+// extent counts there.
+bool skinny_offsets_fit(const GemmCanon &g, const hmlSgemmVariant &v,
+                        const SkinnyPlan &pl) {
   const double lim = 2147483647.0;
+  if (g.form == kFormAxpy) {
+    const double a = (64.0 * v.rt + 16.0 * (double)g.sAk) * 4.0;
+    const double b = (16.0 * v.ct * (double)g.sBn + 16.0 * (double)g.sBk) * 4.0;
+    return a < lim && b < lim;
+  }
   const double rows = (double)std::min<int64_t>(16 * v.rt, g.M);
   const double cols = (double)std::min<int64_t>(16 * v.ct, g.N);
   const double a = ((rows - 1) * (double)g.sAm + pl.kchunk + 4) * 4.0;
@@ -223,8 +222,11 @@ void heuristic_sgemm(const GemmKey &k, hmlSgemmVariant *v) {
     v->wpb = 4;
     tiles = ceil_div(k.M, 16 * v->rt) * ceil_div(nct, v->ct);
   } else {
+    // k_skinny_axpy: 64 rows x 16 ct columns per wave; the 4 waves of a
+    // workgroup split K and sum in LDS.  8 column tiles hold one wave per
+    // SIMD: two blocks of 4 measured faster for N = 128.
     v->family = HML_SGEMM_FAMILY_AXPY;
-    v->ct = nct <= 1 ? 1 : nct <= 2 ? 2 : nct <= 4 ? 4 : 8;
+    v->ct = nct <= 1 ? 1 : nct <= 2 ? 2 : 4;
     v->rt = 1;
     v->wpb = 4;
     tiles = ceil_div(k.M, 64) * ceil_div(nct, v->ct);
@@ -233,9 +235,9 @@ void heuristic_sgemm(const GemmKey &k, hmlSgemmVariant *v) {
   // register and LDS budget lets the device hold at once (from the
   // occupancy calculator), so the whole split runs concurrently; each K
   // chunk at least 256 long.
-  // AXPY form: 2048 waves (measured on 8192 x N x 8192: a 16-way split beat
-  // 32-way by 1.2-1.3x; every split is partials plus a reduction pass)
-  int64_t target = 2048;
+  // AXPY form: 1024 waves (measured on 8192 x N x 8192, N = 2..128: within
+  // 3% of the best variant, where 2048 waves were 5-20% slower)
+  int64_t target = 1024;
   if (k.form == kFormDot) {
     const int rw = skinny_dot_resident_waves(v->rt, v->ct);
     target = rw > 0 ? rw : 3072;
@@ -270,6 +272,9 @@ bool sgemm_variant_runnable(const GemmKey &k, const hmlSgemmVariant &v) {
     return k.form == kFormDot && v.ksplit >= 1 && v.ksplit <= 4096 &&
            (v.wpb == 1 || v.wpb == 2 || v.wpb == 4 || v.wpb == 8) &&
            skinny_dot_runnable(v.rt, v.ct, v.wpb);
+  if (v.family == HML_SGEMM_FAMILY_AXPY)
+    return k.form == kFormAxpy && v.ksplit >= 1 && v.ksplit <= 4096 &&
+           skinny_axpy_runnable(v.rt, v.ct, v.wpb);
   if (v.wpb != 1 && v.wpb != 2 && v.wpb != 4) return false;
   if (v.family == HML_SGEMM_FAMILY_GEMV) {
     if (!k.gemv_ok) return false;
@@ -278,8 +283,6 @@ bool sgemm_variant_runnable(const GemmKey &k, const hmlSgemmVariant &v) {
     return sgemv_variant_runnable(gk, gv);
   }
   if (is_dense_family(v.family)) return true;
-  if (v.ksplit < 1 || v.ksplit > 4096) return false;
-  if (v.family == HML_SGEMM_FAMILY_AXPY) return k.form == kFormAxpy && skinny_has_variant(kFormAxpy, v.rt, v.ct);
   return false;
 }
 
@@ -304,8 +307,7 @@ size_t workspace_bytes(const GemmKey &k, const hmlSgemmVariant &v) {
     return v.ksplit > 1 ? align256((size_t)v.ksplit * k.M * sizeof(float)) : 0;
   }
   const SkinnyPlan pl = plan_skinny(k, v);
-  return align256(pl.bp_floats * sizeof(float)) +
-         align256(pl.p_floats * sizeof(float));
+  return align256(pl.p_floats * sizeof(float));
 }
 
 hmlStatus run_gemv_family(const GemmCanon &g, const GemmKey &k,
@@ -331,17 +333,18 @@ hmlStatus run_sgemm(const GemmCanon &g, const GemmKey &k,
                      read_c, ws, ws ? need : 0, st);
 
   const SkinnyPlan pl = plan_skinny(k, v);
+  if (!skinny_offsets_fit(g, v, pl)) {
+    // strides too large for 31-bit buffer offsets: dense tiles instead
+    // (they address with 64-bit pointers through the split pass)
+    const size_t dn = dense_workspace(0, k.M, k.N, k.K);
+    void *dws = nullptr;
+    hmlStatus s = library_workspace(dn, &dws);
+    if (s != HML_STATUS_SUCCESS) return s;
+    return run_dense(0, g.A, g.sAm, g.sAk, g.B, g.sBk, g.sBn, g.C, g.sCm, g.sCn,
+                     g.M, g.N, g.K, alpha, beta, read_c, dws, dn, st);
+  }
+  hipError_t e;
   if (k.form == kFormDot) {
-    if (!dot_offsets_fit(g, v, pl)) {
-      // strides too large for 31-bit buffer offsets: dense tiles instead
-      // (they address with 64-bit pointers through the split pass)
-      const size_t dn = dense_workspace(0, k.M, k.N, k.K);
-      void *dws = nullptr;
-      hmlStatus s = library_workspace(dn, &dws);
-      if (s != HML_STATUS_SUCCESS) return s;
-      return run_dense(0, g.A, g.sAm, g.sAk, g.B, g.sBk, g.sBn, g.C, g.sCm, g.sCn,
-                       g.M, g.N, g.K, alpha, beta, read_c, dws, dn, st);
-    }
     DotParams p{};
     p.A = g.A; p.sAm = g.sAm;
     p.B = g.B; p.sBk = g.sBk; p.sBn = g.sBn;
@@ -351,39 +354,25 @@ hmlStatus run_sgemm(const GemmCanon &g, const GemmKey &k,
     p.kchunk = pl.kchunk;
     p.kgroups = pl.ksplit;
     p.alpha = alpha; p.beta = beta; p.read_c = read_c;
-    hipError_t e = v.family == HML_SGEMM_FAMILY_DOTB
-                       ? launch_skinny_dotb(v.rt, v.ct, g.sBk == 1, p, st)
-                       : launch_skinny_dot(v.rt, v.ct, g.sBk == 1, pl.kw, p, st);
-    if (e != hipSuccess) return from_hip(e, "skinny kernel launch");
-    if (pl.ksplit > 1) {
-      e = launch_reduce(p.P, pl.ksplit, g.M * g.N, g.N, g.C, g.sCm, g.sCn, alpha,
-                        beta, read_c, st);
-      if (e != hipSuccess) return from_hip(e, "split-K reduce");
-    }
-    return HML_STATUS_SUCCESS;
+    e = v.family == HML_SGEMM_FAMILY_DOTB
+            ? launch_skinny_dotb(v.rt, v.ct, g.sBk == 1, p, st)
+            : launch_skinny_dot(v.rt, v.ct, g.sBk == 1, pl.kw, p, st);
+  } else {
+    SkinnyParams p{};
+    p.A = g.A; p.sAm = g.sAm; p.sAk = g.sAk;
+    p.B = g.B; p.sBk = g.sBk; p.sBn = g.sBn;
+    p.C = g.C; p.sCm = g.sCm; p.sCn = g.sCn;
+    p.P = (float *)ws;
+    p.M = g.M; p.N = g.N; p.K = g.K;
+    p.kchunk = pl.kchunk; p.kgroups = pl.ksplit;
+    p.nct_pad = pl.nct_pad;
+    p.alpha = alpha; p.beta = beta; p.read_c = read_c;
+    e = launch_skinny_axpy(v.rt, v.ct, g.M % 4 == 0, g.sBk == 1 || g.K == 1, pl.kw, p, st);
   }
-  float *Bp = (float *)ws;
-  float *P = (float *)((char *)ws + align256(pl.bp_floats * sizeof(float)));
-  const int64_t nsteps = ceil_div(k.K, kAxpyKStep);
-  hipError_t e = launch_pack_b(g.B, g.sBk, g.sBn, g.K, g.N, nsteps, pl.nct_pad,
-                               Bp, st);
-  if (e != hipSuccess) return from_hip(e, "pack B");
-  SkinnyParams p{};
-  p.A = g.A; p.sAm = g.sAm; p.sAk = g.sAk;
-  p.Bp = Bp;
-  p.C = g.C; p.sCm = g.sCm; p.sCn = g.sCn;
-  p.P = P;
-  p.M = g.M; p.N = g.N; p.K = g.K;
-  p.kchunk = pl.kchunk; p.ksplit = pl.ksplit;
-  p.nct_pad = pl.nct_pad;
-  p.alpha = alpha; p.beta = beta; p.read_c = read_c;
-  const bool vec = ((uintptr_t)g.A % 16 == 0) && (g.sAk % 4 == 0 || g.K == 1) &&
-                   g.M % 4 == 0;
-  e = launch_skinny(k.form, v.rt, v.ct, vec, v.wpb, p, st);
   if (e != hipSuccess) return from_hip(e, "skinny kernel launch");
   if (pl.ksplit > 1) {
-    e = launch_reduce(P, pl.ksplit, g.M * g.N, g.N, g.C, g.sCm, g.sCn, alpha,
-                      beta, read_c, st);
+    e = launch_reduce((const float *)ws, pl.ksplit, g.M * g.N, g.N, g.C, g.sCm, g.sCn,
+                      alpha, beta, read_c, st);
     if (e != hipSuccess) return from_hip(e, "split-K reduce");
   }
   return HML_STATUS_SUCCESS;

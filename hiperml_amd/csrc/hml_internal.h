@@ -14,24 +14,25 @@ enum Form : int {
   kFormAxpy = 1,  // A(m,k) at A[m + k*sAk], contiguous along M
 };
 
-// k-step (K values consumed per loop iteration) of the AXPY form; its packed
-// B layout and K-split granularity are multiples of it.
+// k-step (K values consumed per loop iteration) of the AXPY form; its
+// K-split granularity is a multiple of it.
 constexpr int kAxpyKStep = 16;
 // Rows covered by one "row tile" of each form.
 constexpr int kDotRowsPerTile = 16;
 constexpr int kAxpyRowsPerTile = 64;
 
 struct SkinnyParams {
-  const float *A;   // streamed operand
-  int64_t sAm, sAk; // its strides (one of them is 1)
-  const float *Bp;  // B packed in MFMA fragment order (zero padded)
+  const float *A;   // streamed operand, A(m,k) = A[m + k*sAk]
+  int64_t sAm, sAk; // its strides (sAm == 1)
+  const float *B;   // B(k,n) = B[k*sBk + n*sBn]
+  int64_t sBk, sBn;
   float *C;         // output
   int64_t sCm, sCn;
-  float *P;         // partials [ksplit][M*N] when ksplit > 1
+  float *P;         // partials [kgroups][M*N] when kgroups > 1
   int64_t M, N, K;
-  int64_t kchunk;   // K range per split, multiple of the k-step
-  int32_t ksplit;
-  int32_t nct_pad;  // column tiles held in Bp (multiple of the variant's CT)
+  int64_t kchunk;   // K range per wave, multiple of the k-step
+  int32_t kgroups;  // workgroups along K
+  int32_t nct_pad;  // column tiles covered by the grid (multiple of the variant's CT)
   float alpha, beta;
   int32_t read_c;   // C participates in the epilogue
 };
@@ -49,15 +50,14 @@ struct GemvParams {
   int32_t read_c;
 };
 
-// Packs B(k,n) = B[k*sBk + n*sBn] into the AXPY kernel's fragment order.
-hipError_t launch_pack_b(const float *B, int64_t sBk, int64_t sBn, int64_t K,
-                         int64_t N, int64_t nsteps, int nct_pad, float *Bp,
-                         hipStream_t st);
-size_t packed_b_floats(int64_t K, int nct_pad);
-
-// Returns hipErrorInvalidValue when (form, rt, ct) has no instantiation.
-hipError_t launch_skinny(int form, int rt, int ct, bool vec, int wpb,
-                         const SkinnyParams &p, hipStream_t st);
+// AXPY-form skinny SGEMM (hml_k_skinny.hip, k_skinny_axpy): B read in
+// place, K split over the kw waves of a workgroup and summed there; kgroups
+// > 1 splits K further over workgroups (partials in P, summed by
+// launch_reduce with R = kgroups).  vec: M % 4 == 0; bvec: sBk == 1.
+// Returns hipErrorInvalidValue when (rt, ct, kw) has no instantiation.
+hipError_t launch_skinny_axpy(int rt, int ct, bool vec, bool bvec, int kw,
+                              const SkinnyParams &p, hipStream_t st);
+bool skinny_axpy_runnable(int rt, int ct, int kw);
 bool skinny_has_variant(int form, int rt, int ct);
 
 // DOT-form skinny SGEMM (hml_k_skinny.hip, k_skinny_dot): B read in place,

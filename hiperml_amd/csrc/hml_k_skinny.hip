@@ -17,7 +17,8 @@
 //    place; k_skinny_dotb shares each B stage between the 4 waves of a
 //    workgroup through LDS; k_grouped_dot runs many problems in one
 //    persistent launch.  AXPY form (A contiguous along M): k_skinny_axpy
-//    reads A in operand order and B packed once per call by k_pack_b.
+//    reads A in operand order (4 columns x 256 bytes per load) and B in
+//    place.
 //  * K is split across waves (inside a workgroup, summed in LDS) and, when
 //    rows alone cannot fill 256 CUs, across workgroups, whose partial tiles
 //    are summed in a fixed order by a second kernel (deterministic, no
@@ -35,26 +36,13 @@ __device__ __forceinline__ f4 mfma4(float a, float b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ void store_out(const SkinnyParams &p, int64_t row,
-                                          int64_t col, int ks, float acc) {
-  if (row < p.M && col < p.N) {
-    if (p.ksplit == 1) {
-      float *cp = p.C + row * p.sCm + col * p.sCn;
-      float v = p.alpha * acc;
-      if (p.read_c) v = p.alpha * acc + p.beta * (*cp);
-      *cp = v;
-    } else {
-      p.P[((int64_t)ks * p.M + row) * p.N + col] = acc;
-    }
-  }
-}
-
-// Both kernels stream K in steps.  Steps that lie wholly inside the wave's
-// K range go through a guard-free pipeline that keeps P steps of loads in
-// flight in P named register buffers (the loop is unrolled by P, so no
-// buffer is ever copied and the compiler waits only for the oldest step);
-// a partial last step (K not a multiple of the step) is loaded with bounds
-// checks afterwards.
+// The skinny kernels stream K in steps (stages).  Steps that lie wholly
+// inside the wave's K range go through a pipeline that keeps PS steps of
+// loads in flight in PS named register sets (the loop is unrolled by PS, so
+// no set is ever copied and the compiler waits only for the oldest step);
+// loads of steps past the range go through a descriptor of size 0 (zeros,
+// no memory traffic), and a partial last step (K not a multiple of the
+// step) is loaded with bounds checks afterwards.
 
 // ---------------------------------------------------------------------------
 // DOT form, one problem: k_skinny_dot.
@@ -389,51 +377,89 @@ __global__ __launch_bounds__(256, 2) void k_grouped_dot(const GroupProblem *prob
 }
 
 // ---------------------------------------------------------------------------
-// AXPY form: A(m,k) = A[m + k*sAk].  A row tile is 64 rows: lane (g, r)
-// loads A(m0 + 4r .. m0 + 4r + 3, k) for k = k0 + 4u + g (u = 0..3 per
-// k-step of 16).  MFMA j (j = 0..3) takes element j, so its 16 "rows" are
-// the interleaved rows m0 + 4r + j; its B operand B(k0 + 4u + g, 16c + r)
-// is shared by the 4 MFMAs.  Packed B fragment (step s, tile c) holds, for
-// lane (g, r), B(16s + 4u + g, 16c + r) for u = 0..3.
-// VEC requires M % 4 == 0, so a lane's 4 rows are all in range or all out;
-// out-of-range lanes load row 0 and multiply zeros.  The scalar variant
-// clamps and zeroes each element on its own.
+// AXPY form: A(m,k) = A[m + k*sAk] (contiguous along M).  A wave computes RT
+// row tiles of 64 rows x CT column tiles over its own K chunk; the kw waves
+// of a workgroup take consecutive chunks of the same tiles and sum their
+// results through LDS at the end, in wave order, so a K split needs no
+// second kernel unless it spans workgroups.
+//
+// K advances in steps of 16.  Lane (g, r) loads A(m0 + 4r .. m0 + 4r + 3, k)
+// for k = k0 + 4g + u, u = 0..3 -- per load instruction 4 columns x 256
+// contiguous bytes.  MFMA (u, j) takes element j of load u: its 16 "rows"
+// are m0 + 4r + j (r = 0..15) and its 4 k are k0 + 4g + u (g = 0..3).  Its
+// B operand at lane (g, r) of column tile c is B(k0 + 4g + u, 16c + r); a
+// lane's 4 values (u = 0..3) are consecutive in k, so K-contiguous B (BVEC)
+// is one 16-byte load per column tile and step, other layouts four scalar
+// loads.  B is read in place with its own strides.  Result element i at
+// lane (g, r) of MFMA (u, j), column tile c, is C(m0 + 4(4g + i) + j,
+// 16c + r).
+//
+// Loads go through buffer descriptors rebuilt per step (base = the step's
+// first column), so only one step's extent has to fit the 31-bit offsets.
+// A lane offset of kOOB returns zero without touching memory: rows past M,
+// columns past N and k past the chunk end.  Steps past the chunk load
+// through a descriptor of size 0, so the pipeline needs no guarded drain.
+// VEC (M % 4 == 0): a lane's 4 rows are all in range or all out, one 16-byte
+// load; otherwise 4 scalar loads, each row checked on its own.
 // ---------------------------------------------------------------------------
-template <int RT, int CT, bool VEC>
-__global__ __launch_bounds__(256) void k_skinny_axpy(SkinnyParams p) {
-  constexpr int P = (RT == 1 && CT <= 2) ? 3 : 2;
-  const int lane = threadIdx.x & 63;
-  const int wpb = blockDim.x >> 6;
-  const int64_t task = (int64_t)blockIdx.x * wpb + (threadIdx.x >> 6);
-  const int64_t rows_per_wave = (int64_t)kAxpyRowsPerTile * RT;
-  const int64_t nrb = (p.M + rows_per_wave - 1) / rows_per_wave;
-  const int64_t rb = task / p.ksplit;
-  const int ks = (int)(task - rb * p.ksplit);
-  if (rb >= nrb) return;
-  const int ct0 = blockIdx.y * CT;
-  const int r = lane & 15, g = lane >> 4;
+// register stages of loads in flight, and workgroup size limit, per shape
+__host__ __device__ constexpr int axpy_stages(int rt, int ct) {
+  return (rt == 1 && ct <= 2) ? 3 : 2;
+}
+__host__ __device__ constexpr int axpy_max_waves(int rt, int ct) {
+  return rt * ct >= 8 ? 4 : 8;
+}
+constexpr int kAxpyRedGroups = 4;  // accumulator f4 groups summed per LDS pass
 
-  // per lane: 4 rows m4[t] .. m4[t] + 3 of row tile t
-  int64_t m4[RT];
-  bool rin[RT][4];
-  const float *arow[RT][4];
-#pragma unroll
-  for (int t = 0; t < RT; ++t) {
-    m4[t] = rb * rows_per_wave + t * 64 + 4 * r;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      rin[t][e] = m4[t] + e < p.M;
-      arow[t][e] = p.A + (rin[t][e] ? m4[t] + e : (VEC ? 0 : p.M - 1));
+__device__ __forceinline__ void store_out(const SkinnyParams &p, int64_t row,
+                                          int64_t col, int kgi, float v) {
+  if (row < p.M && col < p.N) {
+    if (p.kgroups == 1) {
+      float *cp = p.C + row * p.sCm + col * p.sCn;
+      float o = p.alpha * v;
+      if (p.read_c) o = p.alpha * v + p.beta * (*cp);
+      *cp = o;
+    } else {
+      p.P[((int64_t)kgi * p.M + row) * p.N + col] = v;
     }
   }
-  const int64_t kb = (int64_t)ks * p.kchunk;
+}
+
+template <int RT, int CT, bool VEC, bool BVEC>
+__global__ __launch_bounds__(64 * axpy_max_waves(RT, CT)) void k_skinny_axpy(SkinnyParams p) {
+  constexpr int PS = axpy_stages(RT, CT);
+  extern __shared__ f4 axpy_lds[];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int kw = blockDim.x >> 6;
+  const int r = lane & 15, g = lane >> 4;
+  const int64_t rb = blockIdx.x / p.kgroups;
+  const int kgi = (int)(blockIdx.x - rb * p.kgroups);
+  const int64_t m0 = rb * kAxpyRowsPerTile * RT;
+  const int ct0 = blockIdx.y * CT;
+  const int64_t kb = ((int64_t)kgi * kw + w) * p.kchunk;
   int64_t ke = kb + p.kchunk;
   if (ke > p.K) ke = p.K;
   const int64_t nfull = ke > kb ? (ke - kb) / kAxpyKStep : 0;
 
-  const f4 *Bp = reinterpret_cast<const f4 *>(p.Bp);
-  const int64_t bstep = (int64_t)p.nct_pad * 64;
-  const f4 *bp = Bp + (kb / kAxpyKStep) * bstep + (int64_t)ct0 * 64 + lane;
+  // lane offsets (bytes) from a step's descriptor bases
+  uint32_t aoff[RT][VEC ? 1 : 4];
+#pragma unroll
+  for (int t = 0; t < RT; ++t)
+#pragma unroll
+    for (int e = 0; e < (VEC ? 1 : 4); ++e) {
+      const int64_t row = 64 * t + 4 * r + e;
+      aoff[t][e] = m0 + row < p.M ? (uint32_t)((row + 4 * g * p.sAk) * 4) : kOOB;
+    }
+  uint32_t boff[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c)
+    boff[c] = (int64_t)(ct0 + c) * 16 + r < p.N
+                  ? (uint32_t)(((16 * c + r) * p.sBn + 4 * g * p.sBk) * 4) : kOOB;
+  const float *Aw = p.A + kb * p.sAk + m0;
+  const float *Bw = p.B + kb * p.sBk + (int64_t)ct0 * 16 * p.sBn;
+  const __amdgpu_buffer_rsrc_t rnull = make_rsrc(p.A, 0);
+  const int64_t astep = (int64_t)kAxpyKStep * p.sAk, bstep = (int64_t)kAxpyKStep * p.sBk;
 
   f4 acc[RT][4][CT];
 #pragma unroll
@@ -443,32 +469,33 @@ __global__ __launch_bounds__(256) void k_skinny_axpy(SkinnyParams p) {
 #pragma unroll
       for (int c = 0; c < CT; ++c) acc[t][j][c] = f4{0.f, 0.f, 0.f, 0.f};
 
-  // A values of row tile t at column k (zero where the row is outside M)
-  auto ldA = [&](int t, int64_t k) -> f4 {
-    f4 v;
-    if (VEC) {
-      v = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(arow[t][0] + k * p.sAk));
-      if (!rin[t][0]) v = f4{0.f, 0.f, 0.f, 0.f};
-    } else {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const float x = __builtin_nontemporal_load(arow[t][e] + k * p.sAk);
-        v[e] = rin[t][e] ? x : 0.f;
-      }
-    }
-    return v;
-  };
-
-  f4 a[P][RT][4], b[P][CT];
+  f4 a[PS][RT][4], b[PS][CT];
   auto load = [&](int j, int64_t s) {
+    const bool live = s < nfull;
+    const __amdgpu_buffer_rsrc_t ra = live ? make_rsrc(Aw + s * astep, kRsrcMax) : rnull;
+    const __amdgpu_buffer_rsrc_t rbs = live ? make_rsrc(Bw + s * bstep, kRsrcMax) : rnull;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int64_t k = kb + s * kAxpyKStep + 4 * u + g;
+      const int32_t so = (int32_t)(u * p.sAk * 4);
 #pragma unroll
-      for (int t = 0; t < RT; ++t) a[j][t][u] = ldA(t, k);
+      for (int t = 0; t < RT; ++t) {
+        if (VEC) {
+          a[j][t][u] = buf_load4<kBufNT>(ra, aoff[t][0], so);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) a[j][t][u][e] = buf_load1<kBufNT>(ra, aoff[t][e], so);
+        }
+      }
     }
 #pragma unroll
-    for (int c = 0; c < CT; ++c) b[j][c] = bp[s * bstep + c * 64];
+    for (int c = 0; c < CT; ++c) {
+      if (BVEC) {
+        b[j][c] = buf_load4<0>(rbs, boff[c], 0);
+      } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) b[j][c][u] = buf_load1(rbs, boff[c], (int32_t)(u * p.sBk * 4));
+      }
+    }
   };
   auto comp = [&](int j) {
 #pragma unroll
@@ -482,87 +509,85 @@ __global__ __launch_bounds__(256) void k_skinny_axpy(SkinnyParams p) {
             acc[t][jj][c] = mfma4(a[j][t][u][jj], b[j][c][u], acc[t][jj][c]);
   };
 
-  if (nfull >= P) {
+  // full steps, PS at a time: step t uses register set t % PS and is loaded
+  // PS steps ahead, A and B of a step together (one in-order load counter)
 #pragma unroll
-    for (int j = 0; j < P; ++j) {
-      load(j, j);
-      __builtin_amdgcn_sched_barrier(0);  // issue the stages in order
-    }
-    int64_t s = 0;
-    for (; s + 2 * P <= nfull; s += P) {
+  for (int j = 0; j < PS; ++j) load(j, j);
+  __builtin_amdgcn_sched_barrier(0);
+  for (int64_t s = 0; s < nfull; s += PS) {
 #pragma unroll
-      for (int j = 0; j < P; ++j) {
-        comp(j);
-        load(j, s + P + j);
-        __builtin_amdgcn_sched_barrier(0);  // keep each stage's loads right after its compute
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < P; ++j) {
-      comp(j);
-      if (s + P + j < nfull) load(j, s + P + j);
-    }
-#pragma unroll
-    for (int j = 0; j < P; ++j)
-      if (s + P + j < nfull) comp(j);
-  } else {
-    for (int64_t s = 0; s < nfull; ++s) {
-      load(0, s);
-      comp(0);
+    for (int j = 0; j < PS; ++j) {
+      if (s + j < nfull) comp(j);
+      load(j, s + PS + j);
+      __builtin_amdgcn_sched_barrier(0);  // keep each step's loads right after its compute
     }
   }
   if (kb + nfull * kAxpyKStep < ke) {
-    // partial last step: k beyond ke reads as zero
+    // partial last step: k >= ke reads as zero (per lane: its k is k0 + 4g + u)
+    const int64_t k0 = kb + nfull * kAxpyKStep;
+    const __amdgpu_buffer_rsrc_t ra = make_rsrc(Aw + nfull * astep, kRsrcMax);
+    const __amdgpu_buffer_rsrc_t rbs = make_rsrc(Bw + nfull * bstep, kRsrcMax);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int64_t k = kb + nfull * kAxpyKStep + 4 * u + g;
+      const bool kin = k0 + 4 * g + u < ke;
+      const int32_t so = (int32_t)(u * p.sAk * 4);
 #pragma unroll
-      for (int t = 0; t < RT; ++t)
-        a[0][t][u] = k < ke ? ldA(t, k) : f4{0.f, 0.f, 0.f, 0.f};
+      for (int t = 0; t < RT; ++t) {
+        if (VEC) {
+          a[0][t][u] = buf_load4<kBufNT>(ra, kin ? aoff[t][0] : kOOB, so);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            a[0][t][u][e] = buf_load1<kBufNT>(ra, kin ? aoff[t][e] : kOOB, so);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < CT; ++c)
+        b[0][c][u] = buf_load1(rbs, kin ? boff[c] : kOOB, (int32_t)(u * p.sBk * 4));
     }
-#pragma unroll
-    for (int c = 0; c < CT; ++c) b[0][c] = bp[nfull * bstep + c * 64];
     comp(0);
   }
 
+  if (kw == 1) {
 #pragma unroll
-  for (int t = 0; t < RT; ++t)
+    for (int t = 0; t < RT; ++t)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int c = 0; c < CT; ++c)
+        for (int c = 0; c < CT; ++c)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int64_t row = rb * rows_per_wave + t * 64 + 4 * (4 * g + i) + j;
-          const int64_t col = (int64_t)(ct0 + c) * 16 + r;
-          store_out(p, row, col, ks, acc[t][j][c][i]);
-        }
-}
-
-// ---------------------------------------------------------------------------
-// B packing for the AXPY form.  One thread writes one 128-bit fragment
-// element: lane (g, r) of (k-step s, column tile c) holds B(16s + 4u + g,
-// 16c + r), u = 0..3, zero past K and N.
-// ---------------------------------------------------------------------------
-__global__ void k_pack_b(const float *__restrict__ B, int64_t sBk, int64_t sBn,
-                         int64_t K, int64_t N, int64_t total, int nct_pad,
-                         f4 *__restrict__ Bp) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= total) return;
-  const int lane = (int)(idx & 63);
-  const int r = lane & 15, g = lane >> 4;
-  const int64_t sc = idx >> 6;  // s*nct_pad + c
-  const int64_t c = sc % nct_pad, s = sc / nct_pad;
-  const int64_t n = c * 16 + r;
-  f4 v = {0.f, 0.f, 0.f, 0.f};
-  if (n < N) {
+          for (int i = 0; i < 4; ++i)
+            store_out(p, m0 + 64 * t + 4 * (4 * g + i) + j, (int64_t)(ct0 + c) * 16 + r, kgi,
+                      acc[t][j][c][i]);
+    return;
+  }
+  // Sum over the waves of the workgroup in wave order, kAxpyRedGroups
+  // accumulator groups (group q = (t*4 + j)*CT + c) per pass: every wave
+  // writes its groups, then wave w sums groups w, w + kw, ... of the pass.
+  constexpr int NG = RT * 4 * CT;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int64_t k = s * kAxpyKStep + g + 4 * j;
-      if (k < K) v[j] = B[k * sBk + n * sBn];
+  for (int pass = 0; pass < NG / kAxpyRedGroups; ++pass) {
+    __syncthreads();  // the previous pass's reads are done
+#pragma unroll
+    for (int i = 0; i < kAxpyRedGroups; ++i) {
+      const int q = pass * kAxpyRedGroups + i;
+      axpy_lds[(w * kAxpyRedGroups + i) * 64 + lane] = acc[q / (4 * CT)][(q / CT) % 4][q % CT];
+    }
+    __syncthreads();
+    for (int i = w; i < kAxpyRedGroups; i += kw) {
+      f4 sum = axpy_lds[i * 64 + lane];
+      for (int ww = 1; ww < kw; ++ww) {
+        const f4 x = axpy_lds[(ww * kAxpyRedGroups + i) * 64 + lane];
+        sum[0] += x[0]; sum[1] += x[1]; sum[2] += x[2]; sum[3] += x[3];
+      }
+      const int q = pass * kAxpyRedGroups + i;
+      const int t = q / (4 * CT), j = (q / CT) % 4, c = q % CT;
+#pragma unroll
+      for (int ii = 0; ii < 4; ++ii)
+        store_out(p, m0 + 64 * t + 4 * (4 * g + ii) + j, (int64_t)(ct0 + c) * 16 + r, kgi,
+                  sum[ii]);
     }
   }
-  Bp[idx] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -938,52 +963,45 @@ int resident_blocks(const void *fn, size_t lds) {
   return std::max(1, nb) * prop.multiProcessorCount;
 }
 
-template <int RT, int CT, bool VEC>
-hipError_t run_axpy(int wpb, const SkinnyParams &p, hipStream_t st) {
+template <int RT, int CT, bool VEC, bool BVEC>
+hipError_t run_axpy(int kw, const SkinnyParams &p, hipStream_t st) {
   const int64_t rows = (int64_t)kAxpyRowsPerTile * RT;
-  const int64_t tasks = ((p.M + rows - 1) / rows) * p.ksplit;
-  const int64_t nblk = (tasks + wpb - 1) / wpb;
+  const int64_t nblk = ((p.M + rows - 1) / rows) * p.kgroups;
   const int64_t ncb = p.nct_pad / CT;
   if (nblk > 0x7fffffff || ncb > 65535) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL((k_skinny_axpy<RT, CT, VEC>), dim3((unsigned)nblk, (unsigned)ncb),
-                     dim3(64 * wpb), 0, st, p);
+  const size_t lds = kw > 1 ? (size_t)kw * kAxpyRedGroups * 64 * sizeof(f4) : 0;
+  hipLaunchKernelGGL((k_skinny_axpy<RT, CT, VEC, BVEC>), dim3((unsigned)nblk, (unsigned)ncb),
+                     dim3(64 * kw), lds, st, p);
   return hipGetLastError();
 }
 
 typedef hipError_t (*RunFn)(int, const SkinnyParams &, hipStream_t);
 
 template <int RT, int CT>
-constexpr RunFn axpy_fn(bool vec) { return vec ? run_axpy<RT, CT, true> : run_axpy<RT, CT, false>; }
+constexpr RunFn axpy_fn(bool vec, bool bvec) {
+  return vec ? (bvec ? run_axpy<RT, CT, true, true> : run_axpy<RT, CT, true, false>)
+             : (bvec ? run_axpy<RT, CT, false, true> : run_axpy<RT, CT, false, false>);
+}
 
-RunFn find_skinny(int form, int rt, int ct, bool vec) {
-#define HML_CT_CASES(FN, RT)                    \
-  switch (ct) {                                 \
-    case 1: return FN<RT, 1>(vec);              \
-    case 2: return FN<RT, 2>(vec);              \
-    case 4: return FN<RT, 4>(vec);              \
-    case 8: return FN<RT, 8>(vec);              \
-    default: return nullptr;                    \
+RunFn find_axpy(int rt, int ct, bool vec, bool bvec) {
+  // 2 x 64 rows x 128 columns would spill: no (2, 8) tile
+  switch (rt * 16 + ct) {
+    case 17: return axpy_fn<1, 1>(vec, bvec);
+    case 18: return axpy_fn<1, 2>(vec, bvec);
+    case 20: return axpy_fn<1, 4>(vec, bvec);
+    case 24: return axpy_fn<1, 8>(vec, bvec);
+    case 33: return axpy_fn<2, 1>(vec, bvec);
+    case 34: return axpy_fn<2, 2>(vec, bvec);
+    case 36: return axpy_fn<2, 4>(vec, bvec);
+    default: return nullptr;
   }
-  if (form == kFormDot) {
-    return nullptr;  // k_skinny_dot: launch_skinny_dot
-  } else if (form == kFormAxpy) {
-    switch (rt) {
-      case 1: HML_CT_CASES(axpy_fn, 1)
-      case 2:
-        if (ct == 8) return nullptr;  // 2 x 64 rows x 128 columns spills
-        HML_CT_CASES(axpy_fn, 2)
-      default: return nullptr;
-    }
-  }
-#undef HML_CT_CASES
-  return nullptr;
 }
 
 }  // namespace
 
 bool skinny_has_variant(int form, int rt, int ct) {
   if (form == kFormDot) return find_dot(rt, ct, true) != nullptr;
-  return find_skinny(form, rt, ct, true) != nullptr;
+  return find_axpy(rt, ct, true, true) != nullptr;
 }
 
 size_t skinny_dot_lds_bytes(int rt, int ct, int kw) {
@@ -1036,28 +1054,17 @@ hipError_t launch_skinny_dotb(int rt, int ct, bool bvec, const DotParams &p, hip
   return find_dotb(rt, ct, bvec)(p, st);
 }
 
-hipError_t launch_skinny(int form, int rt, int ct, bool vec, int wpb,
-                         const SkinnyParams &p, hipStream_t st) {
-  RunFn fn = find_skinny(form, rt, ct, vec);
-  if (!fn || (wpb != 1 && wpb != 2 && wpb != 4)) return hipErrorInvalidValue;
-  if (p.nct_pad % ct != 0 || p.ksplit < 1) return hipErrorInvalidValue;
-  return fn(wpb, p, st);
+bool skinny_axpy_runnable(int rt, int ct, int kw) {
+  return find_axpy(rt, ct, true, true) != nullptr && kw >= 1 && kw <= axpy_max_waves(rt, ct) &&
+         (kw & (kw - 1)) == 0;
 }
 
-size_t packed_b_floats(int64_t K, int nct_pad) {
-  const int64_t nsteps = (K + kAxpyKStep - 1) / kAxpyKStep;
-  return (size_t)nsteps * nct_pad * 64 * 4;
-}
-
-hipError_t launch_pack_b(const float *B, int64_t sBk, int64_t sBn, int64_t K,
-                         int64_t N, int64_t nsteps, int nct_pad, float *Bp,
-                         hipStream_t st) {
-  const int64_t total = nsteps * nct_pad * 64;
-  if (total <= 0) return hipSuccess;
-  const int64_t nblk = (total + 255) / 256;
-  hipLaunchKernelGGL(k_pack_b, dim3((unsigned)nblk), dim3(256), 0, st, B, sBk,
-                     sBn, K, N, total, nct_pad, reinterpret_cast<f4 *>(Bp));
-  return hipGetLastError();
+hipError_t launch_skinny_axpy(int rt, int ct, bool vec, bool bvec, int kw,
+                              const SkinnyParams &p, hipStream_t st) {
+  if (!skinny_axpy_runnable(rt, ct, kw) || p.nct_pad % ct != 0 || p.kgroups < 1 ||
+      p.kchunk % kAxpyKStep != 0)
+    return hipErrorInvalidValue;
+  return find_axpy(rt, ct, vec, bvec)(kw, p, st);
 }
 
 hipError_t launch_reduce(const float *P, int R, int64_t total, int64_t ninner,

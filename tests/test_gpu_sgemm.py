@@ -122,6 +122,45 @@ def test_dotb_variants_exact(hml, tb):
                     np.testing.assert_array_equal(C, R, err_msg=repr(v))
 
 
+AXPY_SHAPES = [(1000, 64, 700), (4097, 33, 1000), (301, 16, 64), (2050, 100, 1500),
+               (64, 5, 63), (130, 48, 4100), (6, 3, 5), (4099, 2, 3000), (777, 100, 129)]
+AXPY_TILES = [(1, 1), (1, 2), (1, 4), (1, 8), (2, 1), (2, 2), (2, 4)]
+
+
+@pytest.mark.parametrize("tb", [False, True], ids=["BN", "BT"])
+def test_axpy_variants_exact(hml, tb):
+    """The AXPY-form kernel (A contiguous along M: row-major A^T) for every
+    instantiated tile shape and workgroup size, K split inside one workgroup
+    and across several, B N-contiguous (row-major K x N) and K-contiguous
+    (transposed), M not a multiple of 4 (scalar A loads), K below one step
+    and ragged: bit-exact on integer data."""
+    rng = np.random.default_rng(23)
+    for (M, N, K) in AXPY_SHAPES:
+        A = int_matrix(rng, (M, K))
+        B = int_matrix(rng, (K, N))
+        C0 = int_matrix(rng, (M, N))
+        R = exact_product(A, B, 1.0, 1.0, C0)
+        As, lda = store(A, "row", True)
+        Bs, ldb = store(B, "row", tb)
+        dA = hml.DeviceBuffer.from_numpy(As)
+        dB = hml.DeviceBuffer.from_numpy(Bs)
+        for rt, ct in AXPY_TILES:
+            for wpb in (1, 2, 4, 8):
+                if wpb == 8 and rt * ct >= 8:
+                    continue
+                for ks in (1, 3, 8):
+                    v = hml.SgemmVariant(2, rt, ct, ks, wpb)
+                    print("case", M, N, K, "BT" if tb else "BN", v, flush=True)
+                    dC = hml.DeviceBuffer.from_numpy(np.ascontiguousarray(C0))
+                    hml.sgemm_ex(M, N, K, dA, dB, dC, 1.0, 1.0, order="row", transA=True,
+                                 transB=tb, lda=lda, ldb=ldb, ldc=N, variant=v)
+                    hml.synchronize()
+                    np.testing.assert_array_equal(dC.to_numpy((M, N)), R, err_msg=repr(v))
+                    dC.free()
+        dA.free()
+        dB.free()
+
+
 SMALL = [1, 2, 3, 5, 8, 13, 16, 17]
 EDGE = [31, 32, 33, 63, 64, 65, 127, 128, 129, 255, 256, 257]
 
@@ -348,3 +387,29 @@ def test_large_stride_offsets(hml):
         hml.synchronize()
         np.testing.assert_array_equal(dC.to_numpy((M, N)), exact_product(A, B))
         dA.free()
+
+
+def test_axpy_large_stride_offsets(hml):
+    """AXPY form (row-major A^T, A(m,k) at k*lda + m) with a column stride
+    just inside the per-step 31-bit offset limit of its buffer addressing
+    (lda = 2^24) and one past it (lda = 2^25: the dense kernel takes over).
+    Integer data, bit-exact; only the valid columns are uploaded."""
+    rng = np.random.default_rng(10)
+    for (M, N, K, lda) in ((70, 3, 20, 1 << 24), (70, 3, 20, 1 << 25)):
+        print("case", M, N, K, lda, flush=True)
+        A = int_matrix(rng, (M, K))
+        B = int_matrix(rng, (K, N))
+        nfl = (K - 1) * lda + M
+        dA = hml.DeviceBuffer(4 * nfl)
+        hml.check(hml.lib.hmlMemsetD8(dA.ptr, 0, 4 * nfl))
+        for k in range(K):
+            col = np.ascontiguousarray(A[:, k])
+            hml.check(hml.lib.hmlMemcpyHtoD(dA.ptr + 4 * k * lda, col.ctypes.data, col.nbytes))
+        dB = hml.DeviceBuffer.from_numpy(B)
+        dC = hml.DeviceBuffer(4 * M * N)
+        hml.sgemm_ex(M, N, K, dA, dB, dC, transA=True, lda=lda)
+        hml.synchronize()
+        np.testing.assert_array_equal(dC.to_numpy((M, N)), exact_product(A, B))
+        dA.free()
+        dB.free()
+        dC.free()

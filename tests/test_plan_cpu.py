@@ -1,9 +1,10 @@
 """Host-side launch planning of the skinny kernels, checked through
 hmlSgemmWorkspaceSize (no GPU needed).
 
-DOT form: K is cut into chunks of a multiple of the 64-long pipeline stage;
-the wpb waves of a workgroup take consecutive chunks and sum them in LDS, so
-partial sums go to the workspace only when the chunks span kg > 1
+Both skinny forms cut K into chunks of a multiple of their K step (DOT: the
+64-long pipeline stage, AXPY: 16); the wpb waves of a workgroup (AXPY: the
+largest power of two up to wpb) take consecutive chunks and sum them in
+LDS, so partial sums go to the workspace only when the chunks span kg > 1
 workgroups: kg * M * N floats (rounded up to 256 bytes)."""
 import ctypes
 import math
@@ -38,15 +39,28 @@ def test_dot_workspace(M, N, K, ks, wpb):
     assert ws(M, N, K, (1, 1, 1, ks, wpb)) == dot_partials(M, N, K, ks, wpb)
 
 
-def test_axpy_workspace_packs_b():
-    # AXPY form (transposed A): packed B (16-k steps x column tiles x 64
-    # lanes x 4 floats) plus partials for every split
-    M, N, K = 4096, 16, 1000
-    nsteps = math.ceil(K / 16)
-    packed = (nsteps * 1 * 64 * 4 * 4 + 255) // 256 * 256
-    kchunk = math.ceil(math.ceil(K / 4) / 16) * 16
-    parts = math.ceil(K / kchunk)
-    assert ws(M, N, K, (2, 1, 1, 4, 4), ta=True) == packed + (parts * M * N * 4 + 255) // 256 * 256
+def axpy_partials(M, N, K, ksplit, wpb):
+    kchunk = math.ceil(math.ceil(K / ksplit) / 16) * 16
+    nchunks = math.ceil(K / kchunk)
+    kw = max(1, min(wpb, nchunks))
+    while kw & (kw - 1):
+        kw -= 1
+    kg = math.ceil(nchunks / kw)
+    return 0 if kg == 1 else (kg * M * N * 4 + 255) // 256 * 256
+
+
+@pytest.mark.parametrize("M,N,K,ks,wpb", [
+    (4096, 16, 1000, 4, 4),    # 4 chunks of 256 in one workgroup: no partials
+    (4096, 16, 1000, 16, 4),   # 16 chunks of 64, 4 workgroups
+    (8192, 2, 8192, 12, 8),    # 12 chunks of 688, 2 workgroups of 8 waves
+    (1000, 7, 130, 3, 2),      # 3 chunks of 48, 2 workgroups
+    (1000, 7, 130, 3, 4),      # 3 chunks, workgroup of 2 waves (power of two)
+    (100, 16, 15, 5, 4),       # K below one step: one chunk
+])
+def test_axpy_workspace(M, N, K, ks, wpb):
+    # AXPY form (transposed A): B is read in place, so the workspace is the
+    # partials of the workgroups along K only
+    assert ws(M, N, K, (2, 1, 1, ks, wpb), ta=True) == axpy_partials(M, N, K, ks, wpb)
 
 
 def test_dot_variant_limits():
