@@ -275,13 +275,13 @@ bool sgemm_variant_runnable(const GemmKey &k, const hmlSgemmVariant &v) {
   if (v.family == HML_SGEMM_FAMILY_AXPY)
     return k.form == kFormAxpy && v.ksplit >= 1 && v.ksplit <= 4096 &&
            skinny_axpy_runnable(v.rt, v.ct, v.wpb);
-  if (v.wpb != 1 && v.wpb != 2 && v.wpb != 4) return false;
   if (v.family == HML_SGEMM_FAMILY_GEMV) {
     if (!k.gemv_ok) return false;
     GemvKey gk{k.form == kFormDot ? 1 : 2, k.M, k.K, k.beta0};
     hmlSgemvVariant gv{gk.family, v.rt, v.ct, v.ksplit, v.wpb};
     return sgemv_variant_runnable(gk, gv);
   }
+  if (v.wpb != 1 && v.wpb != 2 && v.wpb != 4) return false;
   if (is_dense_family(v.family)) return true;
   return false;
 }
@@ -304,7 +304,10 @@ size_t workspace_bytes(const GemmKey &k, const hmlSgemmVariant &v) {
   if (is_dense_family(v.family))
     return dense_workspace(dense_precision(v.family), k.M, k.N, k.K);
   if (v.family == HML_SGEMM_FAMILY_GEMV) {
-    return v.ksplit > 1 ? align256((size_t)v.ksplit * k.M * sizeof(float)) : 0;
+    GemvParams p{};
+    const int slots = gemv_resolve_split(k.form == kFormDot ? HML_SGEMV_FAMILY_DOT : HML_SGEMV_FAMILY_AXPY,
+                                         k.K, v.rt, std::max(1, v.ksplit), v.wpb, &p);
+    return slots > 1 ? align256((size_t)slots * k.M * sizeof(float)) : 0;
   }
   const SkinnyPlan pl = plan_skinny(k, v);
   return align256(pl.p_floats * sizeof(float));
@@ -384,20 +387,15 @@ hmlStatus run_gemv_family(const GemmCanon &g, const GemmKey &k,
   const int family = k.form == kFormDot ? HML_SGEMV_FAMILY_DOT : HML_SGEMV_FAMILY_AXPY;
   GemvParams p = gemv_params_from_canon(g, family, alpha, beta, read_c);
   p.P = (float *)ws;
-  p.split = std::max(1, v.ksplit);
   const int64_t reduced = family == HML_SGEMV_FAMILY_DOT ? p.N : p.M;
   const int64_t outputs = family == HML_SGEMV_FAMILY_DOT ? p.M : p.N;
-  const int64_t gran = family == HML_SGEMV_FAMILY_DOT ? 4 * v.rt : 1;
-  int64_t chunk = ceil_div(std::max<int64_t>(reduced, 1), p.split);
-  chunk = ceil_div(chunk, gran) * gran;
-  p.chunk = chunk;
-  p.split = (int)ceil_div(std::max<int64_t>(reduced, 1), chunk);
+  const int slots = gemv_resolve_split(family, reduced, v.rt, std::max(1, v.ksplit), v.wpb, &p);
   bool vec = ((uintptr_t)p.A % 16 == 0) && (p.lda % 4 == 0) && (p.N % 4 == 0) &&
              (family != HML_SGEMV_FAMILY_DOT || (uintptr_t)p.x % 16 == 0);
   hipError_t e = launch_gemv(family, v.rt, v.ct, vec, v.wpb, p, st);
   if (e != hipSuccess) return from_hip(e, "gemv kernel launch");
-  if (p.split > 1) {
-    e = launch_reduce(p.P, p.split, outputs, 1, p.y, 1, 0, alpha, beta,
+  if (slots > 1) {
+    e = launch_reduce(p.P, slots, outputs, 1, p.y, 1, 0, alpha, beta,
                       read_c, st);
     if (e != hipSuccess) return from_hip(e, "gemv split reduce");
   }

@@ -100,13 +100,16 @@ void heuristic_sgemv(const GemvKey &k, hmlSgemvVariant *v) {
     v->split = (int)std::max<int64_t>(1, std::min<int64_t>(split, 1024));
   } else {
     // Column chunks of at most 128 floats (32 lanes); the rows split so
-    // that about 512 workgroups of 4 waves (2048 waves) run, each split at
-    // least 64 rows long, at most 512 splits.
+    // that about 512 workgroups of 4 waves (2048 waves) run -- 1024 below
+    // 64 MB of A, where each wave's short row range is latency-bound
+    // (4099 x 2049: 18.5 -> 12 us) -- each split at least 64 rows long, at
+    // most 512 splits.
     v->lpr = std::min(32, lpr_for(k.outputs));
     v->rb = 1;
     v->wpb = 4;
     const int64_t ncb = ceil_div(k.outputs, 4 * v->lpr);
-    int64_t split = ceil_div(512, ncb);
+    const int64_t target_wg = (double)k.outputs * (double)k.reduced * 4.0 < 67108864.0 ? 1024 : 512;
+    int64_t split = ceil_div(target_wg, ncb);
     split = std::min<int64_t>(split, std::max<int64_t>(1, k.reduced / 64));
     v->split = (int)std::max<int64_t>(1, std::min<int64_t>(split, 512));
   }
@@ -122,9 +125,28 @@ void basic_sgemv(const GemvKey &k, hmlSgemvVariant *v) {
   v->wpb = 4;
 }
 
+int gemv_resolve_split(int family, int64_t reduced, int lpr, int split, int wpb,
+                       GemvParams *p) {
+  const int64_t gran = family == HML_SGEMV_FAMILY_DOT ? 4 * lpr : 1;
+  const int64_t r = std::max<int64_t>(reduced, 1);
+  int64_t chunk = ceil_div(r, std::max(1, split));
+  chunk = ceil_div(chunk, gran) * gran;
+  p->chunk = chunk;
+  p->split = (int)ceil_div(r, chunk);
+  p->wsum = 0;
+  if (family == HML_SGEMV_FAMILY_DOT && wpb > 1 && p->split > 1) {
+    // whole workgroups per row block; trailing chunks past the row are
+    // empty and add zero
+    p->split = (int)(ceil_div(p->split, wpb) * wpb);
+    p->wsum = 1;
+    return p->split / wpb;
+  }
+  return p->split;
+}
+
 bool sgemv_variant_runnable(const GemvKey &k, const hmlSgemvVariant &v) {
   if (v.family != k.family) return false;
-  if (v.wpb != 1 && v.wpb != 2 && v.wpb != 4) return false;
+  if (!gemv_wpb_ok(v.family, v.wpb)) return false;
   if (v.split < 1 || v.split > 65535) return false;
   return gemv_has_variant(v.family, v.lpr, v.rb);
 }
@@ -138,7 +160,9 @@ void select_sgemv(const GemvKey &k, hmlSgemvVariant *v, int *source) {
 namespace {
 
 size_t gemv_workspace(const GemvKey &k, const hmlSgemvVariant &v) {
-  return v.split > 1 ? align256((size_t)v.split * k.outputs * sizeof(float)) : 0;
+  GemvParams p{};
+  const int slots = gemv_resolve_split(k.family, k.reduced, v.lpr, v.split, v.wpb, &p);
+  return slots > 1 ? align256((size_t)slots * k.outputs * sizeof(float)) : 0;
 }
 
 hmlStatus sgemv_impl(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
@@ -181,12 +205,8 @@ hmlStatus sgemv_impl(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
   }
   GemvParams p = c.p;
   p.alpha = alpha; p.beta = beta; p.read_c = read_c;
-  const int64_t gran = c.family == HML_SGEMV_FAMILY_DOT ? 4 * v.lpr : 1;
-  int64_t chunk = ceil_div(c.reduced, v.split);
-  chunk = ceil_div(chunk, gran) * gran;
-  p.chunk = chunk;
-  p.split = (int)ceil_div(c.reduced, chunk);
-  const size_t need = p.split > 1 ? align256((size_t)p.split * c.outputs * sizeof(float)) : 0;
+  const int slots = gemv_resolve_split(c.family, c.reduced, v.lpr, v.split, v.wpb, &p);
+  const size_t need = slots > 1 ? align256((size_t)slots * c.outputs * sizeof(float)) : 0;
   if (need > 0) {
     if (!ws) {
       hmlStatus s = library_workspace(need, &ws);
@@ -202,8 +222,8 @@ hmlStatus sgemv_impl(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
                    (c.family != HML_SGEMV_FAMILY_DOT || (uintptr_t)x % 16 == 0);
   hipError_t e = launch_gemv(v.family, v.lpr, v.rb, vec, v.wpb, p, st);
   if (e != hipSuccess) return from_hip(e, "gemv kernel launch");
-  if (p.split > 1) {
-    e = launch_reduce(p.P, p.split, c.outputs, 1, y, 1, 0, alpha, beta, read_c, st);
+  if (slots > 1) {
+    e = launch_reduce(p.P, slots, c.outputs, 1, y, 1, 0, alpha, beta, read_c, st);
     if (e != hipSuccess) return from_hip(e, "gemv split reduce");
   }
   return HML_STATUS_SUCCESS;

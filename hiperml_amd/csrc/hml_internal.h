@@ -46,9 +46,14 @@ struct GemvParams {
   int64_t M, N;     // DOT: M outputs, rows of length N; AXPY: M reduced rows, N outputs
   int64_t chunk;    // length of the reduced dimension per split
   int32_t split;
+  int32_t wsum;     // DOT: the waves of a workgroup take consecutive chunks of
+                    // the same rows and sum them in LDS (split % wpb == 0);
+                    // partials [split / wpb][outputs] when split > wpb
   float alpha, beta;
   int32_t read_c;
 };
+// Most waves per workgroup of the GEMV DOT kernels.
+constexpr int kGemvMaxWaves = 8;
 
 // AXPY-form skinny SGEMM (hml_k_skinny.hip, k_skinny_axpy): B read in
 // place, K split over the kw waves of a workgroup and summed there; kgroups
@@ -107,6 +112,13 @@ hipError_t launch_scale(float *C, int64_t M, int64_t N, int64_t sCm,
 hipError_t launch_gemv(int family, int lpr, int rb, bool vec, int wpb,
                        const GemvParams &p, hipStream_t st);
 bool gemv_has_variant(int family, int lpr, int rb);
+// Workgroup sizes (waves) the family's kernels take: 1, 2, 4, and 8 for DOT.
+bool gemv_wpb_ok(int family, int wpb);
+// Resolves a GEMV variant's split of `reduced` into p->chunk, p->split and
+// p->wsum; returns the partial slices the launch writes (1: none, y
+// directly; > 1: the split reduction sums that many).
+int gemv_resolve_split(int family, int64_t reduced, int lpr, int split, int wpb,
+                       GemvParams *p);
 
 struct CheckParams {
   const float *A; int64_t sAm, sAk;

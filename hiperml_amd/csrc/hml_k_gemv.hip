@@ -10,7 +10,9 @@
 //         with 128-bit loads, LPR lanes side by side, and reduces with DPP
 //         row operations (quad_perm, row_half_mirror, row_mirror) plus two
 //         cross-row shuffles.  Each lane group handles RB rows so one x load
-//         serves RB rows.  Optional split of n across waves with partials.
+//         serves RB rows.  Optional split of n across waves: the waves of a
+//         workgroup sum their chunks in LDS, partials only for splits wider
+//         than a workgroup.
 //  AXPY : y[n] = sum_m A[m*lda + n] x[m]  (each output a column sum).
 //         LPR lanes x 4 floats span a column chunk; the 64/LPR lane groups
 //         and the waves of a block take interleaved rows; lane groups fold
@@ -62,8 +64,57 @@ __device__ __forceinline__ f4 load4(const float *p, int64_t nvalid) {
   return v;
 }
 
+// End of a DOT wave: its lane groups' sums of RB rows (row0 + iG + grp) go
+// to y, or to slice `sp` of the partials.  With p.wsum the waves of the
+// workgroup hold consecutive chunks of the same rows: they meet in LDS and
+// wave 0 adds them in wave order, writing slice sp / wpb (no partials at all
+// when one workgroup covers the whole row).
+template <int LPR, int RB>
+__device__ __forceinline__ void dot_finish(const GemvParams &p, const float (&acc)[RB],
+                                           int64_t row0, int sp) {
+  constexpr int G = 64 / LPR;
+  __shared__ float red[kGemvMaxWaves * RB * G];
+  const int lane = threadIdx.x & 63;
+  const int grp = lane / LPR, q = lane % LPR;
+  float s[RB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i) s[i] = group_sum<LPR>(acc[i]);
+  int slot = sp, nslot = p.split;
+  if (p.wsum) {
+    const int w = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+    if (q == 0) {
+#pragma unroll
+      for (int i = 0; i < RB; ++i) red[(w * RB + i) * G + grp] = s[i];
+    }
+    __syncthreads();
+    if (w != 0) return;
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      float t = red[i * G + grp];
+      for (int ww = 1; ww < wpb; ++ww) t += red[(ww * RB + i) * G + grp];
+      s[i] = t;
+    }
+    slot = sp / wpb;
+    nslot = p.split / wpb;
+  }
+#pragma unroll
+  for (int i = 0; i < RB; ++i) {
+    const int64_t row = row0 + (int64_t)i * G + grp;
+    if (q == 0 && row < p.M) {
+      if (nslot == 1) {
+        float *yp = p.y + row;
+        float v = p.alpha * s[i];
+        if (p.read_c) v = p.alpha * s[i] + p.beta * (*yp);
+        *yp = v;
+      } else {
+        p.P[(int64_t)slot * p.M + row] = s[i];
+      }
+    }
+  }
+}
+
 template <int LPR, int RB, bool VEC>
-__global__ __launch_bounds__(256) void k_gemv_dot(GemvParams p) {
+__global__ __launch_bounds__(64 * kGemvMaxWaves) void k_gemv_dot(GemvParams p) {
   constexpr int G = 64 / LPR;
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
@@ -72,7 +123,7 @@ __global__ __launch_bounds__(256) void k_gemv_dot(GemvParams p) {
   const int64_t nrb = (p.M + rows_per_wave - 1) / rows_per_wave;
   const int64_t rb = task / p.split;
   const int sp = (int)(task - rb * p.split);
-  if (rb >= nrb) return;  // no barriers in this kernel
+  if (rb >= nrb) return;  // with p.wsum the whole workgroup shares rb
   const int grp = lane / LPR, q = lane % LPR;
 
   const float *arow[RB];
@@ -105,20 +156,7 @@ __global__ __launch_bounds__(256) void k_gemv_dot(GemvParams p) {
     }
   }
 
-#pragma unroll
-  for (int i = 0; i < RB; ++i) {
-    const float s = group_sum<LPR>(acc[i]);
-    if (q == 0 && row[i] < p.M) {
-      if (p.split == 1) {
-        float *yp = p.y + row[i];
-        float v = p.alpha * s;
-        if (p.read_c) v = p.alpha * s + p.beta * (*yp);
-        *yp = v;
-      } else {
-        p.P[(int64_t)sp * p.M + row[i]] = s;
-      }
-    }
-  }
+  dot_finish<LPR, RB>(p, acc, rb * rows_per_wave, sp);
 }
 
 template <int LPR, bool VEC>
@@ -188,7 +226,7 @@ __global__ __launch_bounds__(256) void k_gemv_axpy(GemvParams p) {
 // step uses clamped addresses with zeroed values.
 // ---------------------------------------------------------------------------
 template <int LPR, int RB>
-__global__ __launch_bounds__(256) void k_gemv_dot_stream(GemvParams p) {
+__global__ __launch_bounds__(64 * kGemvMaxWaves) void k_gemv_dot_stream(GemvParams p) {
   constexpr int G = 64 / LPR;
   constexpr int P = RB <= 2 ? 4 : 2;
   constexpr int STEP = 4 * LPR;
@@ -199,7 +237,7 @@ __global__ __launch_bounds__(256) void k_gemv_dot_stream(GemvParams p) {
   const int64_t nrb = (p.M + rows_per_wave - 1) / rows_per_wave;
   const int64_t rb = task / p.split;
   const int sp = (int)(task - rb * p.split);
-  if (rb >= nrb) return;  // no barriers in this kernel
+  if (rb >= nrb) return;  // with p.wsum the whole workgroup shares rb
   const int grp = lane / LPR, q = lane % LPR;
 
   const int64_t cb = (int64_t)sp * p.chunk;
@@ -287,20 +325,7 @@ __global__ __launch_bounds__(256) void k_gemv_dot_stream(GemvParams p) {
     }
   }
 
-#pragma unroll
-  for (int i = 0; i < RB; ++i) {
-    const float s = group_sum<LPR>(acc[i]);
-    if (q == 0 && row[i] < p.M) {
-      if (p.split == 1) {
-        float *yp = p.y + row[i];
-        float v = p.alpha * s;
-        if (p.read_c) v = p.alpha * s + p.beta * (*yp);
-        *yp = v;
-      } else {
-        p.P[(int64_t)sp * p.M + row[i]] = s;
-      }
-    }
-  }
+  dot_finish<LPR, RB>(p, acc, rb * rows_per_wave, sp);
 }
 
 template <int LPR>
@@ -417,7 +442,7 @@ __global__ __launch_bounds__(256) void k_gemv_axpy_stream(GemvParams p) {
 // below 2^31 (gemv_buf_fits).
 // ---------------------------------------------------------------------------
 template <int LPR, int RB>
-__global__ __launch_bounds__(256) void k_gemv_dot_buf(GemvParams p) {
+__global__ __launch_bounds__(64 * kGemvMaxWaves) void k_gemv_dot_buf(GemvParams p) {
   constexpr int G = 64 / LPR;
   constexpr int P = RB <= 2 ? 4 : 2;
   constexpr int STEP = 4 * LPR;
@@ -429,7 +454,7 @@ __global__ __launch_bounds__(256) void k_gemv_dot_buf(GemvParams p) {
   const int64_t nrb = (p.M + rows_per_wave - 1) / rows_per_wave;
   const int64_t rb = task / p.split;
   const int sp = (int)(task - rb * p.split);
-  if (rb >= nrb) return;  // no barriers in this kernel
+  if (rb >= nrb) return;  // with p.wsum the whole workgroup shares rb
   const int grp = lane / LPR, q = lane % LPR;
 
   const int64_t cb = (int64_t)sp * p.chunk;
@@ -511,21 +536,7 @@ __global__ __launch_bounds__(256) void k_gemv_dot_buf(GemvParams p) {
     }
   }
 
-#pragma unroll
-  for (int i = 0; i < RB; ++i) {
-    const float s = group_sum<LPR>(acc[i]);
-    const int64_t row = row0 + i * G + grp;
-    if (q == 0 && row < p.M) {
-      if (p.split == 1) {
-        float *yp = p.y + row;
-        float v = p.alpha * s;
-        if (p.read_c) v = p.alpha * s + p.beta * (*yp);
-        *yp = v;
-      } else {
-        p.P[(int64_t)sp * p.M + row] = s;
-      }
-    }
-  }
+  dot_finish<LPR, RB>(p, acc, row0, sp);
 }
 
 template <int LPR>
@@ -726,6 +737,10 @@ RunFn find_gemv(int family, int lpr, int rb, bool vec) {
 
 }  // namespace
 
+bool gemv_wpb_ok(int family, int wpb) {
+  return wpb == 1 || wpb == 2 || wpb == 4 || (family == 1 && wpb == kGemvMaxWaves);
+}
+
 bool gemv_has_variant(int family, int lpr, int rb) {
   return find_gemv(family, lpr, rb, true) != nullptr;
 }
@@ -733,7 +748,7 @@ bool gemv_has_variant(int family, int lpr, int rb) {
 hipError_t launch_gemv(int family, int lpr, int rb, bool vec, int wpb,
                        const GemvParams &p, hipStream_t st) {
   RunFn fn = find_gemv(family, lpr, rb, vec);
-  if (!fn || (wpb != 1 && wpb != 2 && wpb != 4) || p.split < 1)
+  if (!fn || !gemv_wpb_ok(family, wpb) || p.split < 1 || (p.wsum && p.split % wpb != 0))
     return hipErrorInvalidValue;
   return fn(wpb, p, st);
 }

@@ -66,8 +66,10 @@ typedef enum {
  * reduction pass; ct <= 4 and rt * ct <= 8.  DOTB: the 4 waves (wpb = 4)
  * of a workgroup take 4 consecutive row tiles and share each stage of B
  * through LDS; every K chunk is a workgroup and splits are summed by a
- * reduction pass; rt in {1, 2}, ct in {1, 2, 4}.  AXPY: wpb (1, 2 or 4) waves per workgroup take
- * different tiles; every K split is summed by a reduction pass. */
+ * reduction pass; rt in {1, 2}, ct in {1, 2, 4}.  AXPY: like DOT, the wpb
+ * (1, 2, 4, or 8 when rt * ct < 8) waves of a workgroup take consecutive
+ * chunks of one tile and sum them in LDS; rt in {1, 2}, ct in {1, 2, 4, 8},
+ * not 2 x 8. */
 typedef struct {
   int32_t family; /* hmlSgemmFamily                                          */
   int32_t rt;     /* row tiles per wave                                       */
@@ -82,11 +84,16 @@ typedef enum {
   HML_SGEMV_FAMILY_AXPY  = 2  /* outputs are sums of contiguous rows         */
 } hmlSgemvFamily;
 
+/* SGEMV variants.  DOT with split > 1 and wpb > 1: the wpb waves of a
+ * workgroup take consecutive chunks of the same rows and sum them in LDS
+ * (split is rounded up to a multiple of wpb), so only split / wpb > 1
+ * leaves partials for a reduction pass.  wpb is 1, 2, 4 or 8 (DOT), 1, 2
+ * or 4 (AXPY). */
 typedef struct {
   int32_t family; /* hmlSgemvFamily                                      */
   int32_t lpr;    /* lanes per row (DOT) / lanes per column chunk (AXPY)  */
   int32_t rb;     /* rows per lane group (DOT) / unroll (AXPY)            */
-  int32_t split;  /* split of the reduced dimension across workgroups     */
+  int32_t split;  /* chunks of the reduced dimension                      */
   int32_t wpb;    /* waves per workgroup                                  */
 } hmlSgemvVariant;
 

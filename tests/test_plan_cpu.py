@@ -114,3 +114,29 @@ def test_heuristic_fp32_bound_split():
     assert big.family == 1 and big.ksplit <= big.wpb
     thin = sel(1024, 64, 65536)
     assert thin.family == 1 and thin.ksplit > thin.wpb
+
+
+@pytest.mark.parametrize("variant,slices", [
+    ((1, 64, 2, 8, 8), 0),      # one workgroup covers each row: no partials
+    ((1, 64, 2, 8, 2), 4),      # 8 chunks, 2 per workgroup: 4 slices
+    ((1, 64, 2, 3, 2), 2),      # 3 chunks rounded up to 4: 2 slices
+    ((1, 64, 2, 1, 4), 0),      # no split
+    ((2, 32, 1, 8, 4), 8),      # AXPY: every split is a slice
+])
+def test_gemv_workspace_in_workgroup_sum(variant, slices):
+    """SGEMV DOT splits: the waves of a workgroup sum their chunks in LDS,
+    so the partials hold split / wpb slices of the 16384 outputs."""
+    v = hml.SgemvVariant(*variant)
+    trans = 1 if variant[0] == 2 else 0
+    got = hml.lib.hmlSgemvWorkspaceSize(0, trans, 16384, 16384, ctypes.byref(v))
+    assert got == (0 if slices <= 1 else slices * 16384 * 4)
+
+
+def test_gemv_axpy_small_split_target():
+    """Row-major A^T x below 64 MB of A aims at about 1024 workgroups
+    (latency-bound short row ranges), larger problems at 512."""
+    small = hml.sgemv_select(4099, 2049, trans=True)[0]
+    big = hml.sgemv_select(16384, 16384, trans=True)[0]
+    ncb = lambda n, lpr: -(-n // (4 * lpr))
+    assert small.family == 2 and ncb(2049, small.lpr) * small.split >= 1000
+    assert big.family == 2 and ncb(16384, big.lpr) * big.split <= 600
