@@ -136,10 +136,15 @@ __device__ __forceinline__ void store_c(const DenseParams &p, int64_t row,
 
 // ---- split-precision bf16 GEMM ---------------------------------------------
 // bf16 element offset of 16-byte chunk kq (0..3) of LDS tile row x (32 bf16
-// per row, unpadded): 16 consecutive rows reading one chunk then cover all
-// 64 banks, and 4 threads writing one row's 4 chunks stay contiguous.
+// = 64 bytes per row, unpadded).  The chunk is XORed with
+// f(x) = ((x >> 1) ^ (x >> 2)) & 3, which takes 4 distinct values both over
+// rows of one parity among 8 consecutive rows and over rows of one residue
+// mod 4 among 16: 8 (16) consecutive rows reading one chunk then hit 8 (16)
+// distinct 16-byte bank groups, whether the LDS serves 128 or 256 bytes per
+// clock, and 4 threads writing one row's 4 chunks stay contiguous.  (With
+// f(x) = (x >> 2) & 3, rows x and x + 2 collide in the 128-byte case.)
 __device__ __forceinline__ int sw_off(int x, int kq) {
-  return x * kSplitKBlock + 8 * (kq ^ ((x >> 2) & 3));
+  return x * kSplitKBlock + 8 * (kq ^ (((x >> 1) ^ (x >> 2)) & 3));
 }
 
 template <int S>
