@@ -156,6 +156,15 @@ std::vector<hmlSgemmVariant> sgemm_candidates(const GemmKey &k) {
     hmlSgemmVariant v;
     if (selector_variant(k, c, &v)) push_unique(&out, v);
   }
+  // N = 2, 4, 8 on the multi-vector GEMV kernels: the DOT SGEMV parameter
+  // sweep of the same rows (lanes per row 16..64, 1..4 rows per group).
+  if (k.form == kFormDot && (k.N == 2 || k.N == 4 || k.N == 8)) {
+    GemvKey gk{HML_SGEMV_FAMILY_DOT, k.M, k.K, k.beta0};
+    for (const hmlSgemvVariant &g : sgemv_candidates(gk)) {
+      hmlSgemmVariant v{HML_SGEMM_FAMILY_GEMVN, g.lpr, g.rb, g.split, g.wpb};
+      if (sgemm_variant_runnable(k, v)) push_unique(&out, v);
+    }
+  }
   // Shared-B DOT tiles for wide column blocks (N > 32), K split 1..16.
   if (k.form == kFormDot && k.N > 32) {
     for (int rt : {1, 2})

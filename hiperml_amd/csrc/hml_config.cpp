@@ -344,7 +344,7 @@ bool v1_sgemm_variant(const GemmKey &k, hmlSgemmVariant *v) {
     return true;
   }
   heuristic_sgemm(k, v);
-  if (v->family == HML_SGEMM_FAMILY_GEMV) return true;
+  if (v->family == HML_SGEMM_FAMILY_GEMV || v->family == HML_SGEMM_FAMILY_GEMVN) return true;
   const int ctcap = k.form == kFormDot ? 4 : 8;
   int ct = 1;
   while (ct < ctcap && ct < cs) ct <<= 1;

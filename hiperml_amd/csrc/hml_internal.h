@@ -55,6 +55,30 @@ struct GemvParams {
 // Most waves per workgroup of the GEMV DOT kernels.
 constexpr int kGemvMaxWaves = 8;
 
+// Several right-hand sides on the GEMV DOT structure (k_gemv_multi):
+// C(m, n) = alpha * sum_k A(m,k) B(k,n) + beta * C(m, n), n < NV columns.
+struct GemvNParams {
+  const float *A;   // A(m,k) = A[m*lda + k]
+  int64_t lda;
+  const float *B;   // B(k,n) = B[k*sBk + n*sBn]
+  int64_t sBk, sBn;
+  float *C;
+  int64_t sCm, sCn;
+  float *P;         // partials [slices][M*NV] when the split needs them
+  int64_t M, K;
+  int64_t chunk;    // K per split
+  int32_t split;
+  int32_t wsum;     // the waves of a workgroup sum their chunks in LDS
+  float alpha, beta;
+  int32_t read_c;
+};
+// Instantiated: lpr in {16, 32, 64}, rb in {1, 2, 4}, nv in {2, 4, 8}, wpb
+// in {1, 2, 4, 8}; bvec = B K-contiguous (sBk == 1).
+bool gemv_multi_has_variant(int lpr, int rb, int nv);
+bool gemv_multi_offsets_fit(int lpr, int rb, int nv, bool bvec, const GemvNParams &p);
+hipError_t launch_gemv_multi(int lpr, int rb, int nv, bool bvec, int wpb, const GemvNParams &p,
+                             hipStream_t st);
+
 // AXPY-form skinny SGEMM (hml_k_skinny.hip, k_skinny_axpy): B read in
 // place, K split over the kw waves of a workgroup and summed there; kgroups
 // > 1 splits K further over workgroups (partials in P, summed by

@@ -655,6 +655,234 @@ __global__ __launch_bounds__(256) void k_gemv_axpy_buf(GemvParams p) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Several right-hand sides (skinny SGEMM with NV = 2, 4 or 8 columns, A
+// contiguous along K): C(m, n) = sum_k A(m, k) B(k, n).  The DOT streaming
+// structure with NV accumulators per row: lane groups stream rows of A with
+// 128-bit buffer loads (any dword alignment); per 4-k step a lane loads
+// B(k .. k+3, 0 .. NV-1) once for its RB rows -- NV 16-byte loads along k
+// when B is K-contiguous (BVEC), else four rows of NV contiguous floats --
+// and does 4 NV FMAs per row.  That is NV FMAs per element of A, within the
+// VALU rate at HBM speed up to NV = 8, with no LDS staging or matrix-core
+// padding of the columns (a 16-column MFMA tile for 2 columns).  Loads of
+// steps past the chunk go through a size-0 descriptor (zeros, no traffic),
+// so the pipeline needs no guarded drain; a partial last step is read
+// element by element.  K splits follow the GEMV DOT rules (wsum: the waves
+// of a workgroup add their chunks in LDS in wave order).
+// ---------------------------------------------------------------------------
+template <int LPR, int RB, int NV, bool BVEC>
+__global__ __launch_bounds__(64 * kGemvMaxWaves) void k_gemv_multi(GemvNParams p) {
+  constexpr int G = 64 / LPR;
+  constexpr int PS = RB * NV <= 8 ? 3 : 2;  // steps of loads in flight
+  constexpr int STEP = 4 * LPR;
+  __shared__ float red[kGemvMaxWaves * RB * G * NV];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wpb = blockDim.x >> 6;
+  const int64_t task = (int64_t)blockIdx.x * wpb + w;
+  const int64_t rows_per_wave = (int64_t)G * RB;
+  const int64_t nrb = (p.M + rows_per_wave - 1) / rows_per_wave;
+  const int64_t rb = task / p.split;
+  const int sp = (int)(task - rb * p.split);
+  if (rb >= nrb) return;  // with p.wsum the whole workgroup shares rb
+  const int grp = lane / LPR, q = lane % LPR;
+
+  const int64_t cb = (int64_t)sp * p.chunk;
+  int64_t ce = cb + p.chunk;
+  if (ce > p.K) ce = p.K;
+  const int64_t nfull = ce > cb ? (ce - cb) / STEP : 0;
+  const int64_t row0 = rb * rows_per_wave;
+
+  const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.A + row0 * p.lda + cb, kRsrcMax);
+  const __amdgpu_buffer_rsrc_t rbs = make_rsrc(p.B + cb * p.sBk, kRsrcMax);
+  const __amdgpu_buffer_rsrc_t rnull = make_rsrc(p.A, 0);
+  uint32_t aoff[RB];  // row row0 + iG + grp, k = cb + 4q
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+    aoff[i] = row0 + i * G + grp < p.M
+                  ? (uint32_t)((((int64_t)i * G + grp) * p.lda + 4 * q) * 4) : kOOB;
+  // B(cb + 4q + e, n): BVEC column n at boff[n] (e along the 16 bytes),
+  // else row e at boff[e] (n along the row)
+  uint32_t boff[BVEC ? NV : 4];
+#pragma unroll
+  for (int i = 0; i < (BVEC ? NV : 4); ++i)
+    boff[i] = BVEC ? (uint32_t)((i * p.sBn + 4 * q) * 4) : (uint32_t)((4 * q + i) * p.sBk * 4);
+
+  float acc[RB][NV];
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int n = 0; n < NV; ++n) acc[i][n] = 0.f;
+
+  f4 ba[PS][RB];
+  float bx[PS][4][NV];
+  auto load = [&](int j, int64_t s) {
+    const bool live = s < nfull;
+    const __amdgpu_buffer_rsrc_t rs_a = live ? ra : rnull;
+    const __amdgpu_buffer_rsrc_t rs_b = live ? rbs : rnull;
+    const int32_t so = (int32_t)(s * STEP * 4);
+#pragma unroll
+    for (int i = 0; i < RB; ++i) ba[j][i] = buf_load4<kBufNT>(rs_a, aoff[i], so);
+    if (BVEC) {
+#pragma unroll
+      for (int n = 0; n < NV; ++n) {
+        const f4 t = buf_load4<0>(rs_b, boff[n], so);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) bx[j][e][n] = t[e];
+      }
+    } else {
+      const int32_t sob = (int32_t)(s * STEP * p.sBk * 4);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (NV == 2) {
+          const f2 t = buf_load2(rs_b, boff[e], sob);
+          bx[j][e][0] = t[0];
+          bx[j][e][1] = t[1];
+        } else {
+#pragma unroll
+          for (int h = 0; h < NV / 4; ++h) {
+            const f4 t = buf_load4<0>(rs_b, boff[e] + 16 * h, sob);
+#pragma unroll
+            for (int n = 0; n < 4; ++n) bx[j][e][4 * h + n] = t[n];
+          }
+        }
+      }
+    }
+  };
+  auto comp = [&](int j) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int n = 0; n < NV; ++n) acc[i][n] += ba[j][i][e] * bx[j][e][n];
+  };
+
+#pragma unroll
+  for (int j = 0; j < PS; ++j) load(j, j);
+  __builtin_amdgcn_sched_barrier(0);
+  for (int64_t s = 0; s < nfull; s += PS) {
+#pragma unroll
+    for (int j = 0; j < PS; ++j) {
+      if (s + j < nfull) comp(j);
+      load(j, s + PS + j);
+      __builtin_amdgcn_sched_barrier(0);  // keep each step's loads right after its compute
+    }
+  }
+  if (cb + nfull * STEP < ce) {
+    // last partial step, element by element (past ce: out-of-range offset)
+    const int64_t c = nfull * STEP + 4 * q;  // relative to cb
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const bool in = cb + c + e < ce;
+      float xv[NV];
+#pragma unroll
+      for (int n = 0; n < NV; ++n)
+        xv[n] = buf_load1(rbs, in ? (uint32_t)(((c + e) * p.sBk + n * p.sBn) * 4) : kOOB, 0);
+#pragma unroll
+      for (int i = 0; i < RB; ++i) {
+        const uint32_t o = in && aoff[i] != kOOB ? aoff[i] + (uint32_t)((nfull * STEP + e) * 4) : kOOB;
+        const float av = buf_load1(ra, o, 0);
+#pragma unroll
+        for (int n = 0; n < NV; ++n) acc[i][n] += av * xv[n];
+      }
+    }
+  }
+
+  // lane groups' sums; with p.wsum the waves of the workgroup meet in LDS
+  // and wave 0 adds them in wave order
+  float sm[RB][NV];
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int n = 0; n < NV; ++n) sm[i][n] = group_sum<LPR>(acc[i][n]);
+  int slot = sp, nslot = p.split;
+  if (p.wsum) {
+    if (q == 0) {
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int n = 0; n < NV; ++n) red[((w * RB + i) * G + grp) * NV + n] = sm[i][n];
+    }
+    __syncthreads();
+    if (w != 0) return;
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+      for (int n = 0; n < NV; ++n) {
+        float t = red[(i * G + grp) * NV + n];
+        for (int ww = 1; ww < wpb; ++ww) t += red[((ww * RB + i) * G + grp) * NV + n];
+        sm[i][n] = t;
+      }
+    slot = sp / wpb;
+    nslot = p.split / wpb;
+  }
+#pragma unroll
+  for (int i = 0; i < RB; ++i) {
+    const int64_t row = row0 + (int64_t)i * G + grp;
+    if (q == 0 && row < p.M) {
+#pragma unroll
+      for (int n = 0; n < NV; ++n) {
+        if (nslot == 1) {
+          float *cp = p.C + row * p.sCm + n * p.sCn;
+          float v = p.alpha * sm[i][n];
+          if (p.read_c) v = p.alpha * sm[i][n] + p.beta * (*cp);
+          *cp = v;
+        } else {
+          p.P[((int64_t)slot * p.M + row) * NV + n] = sm[i][n];
+        }
+      }
+    }
+  }
+}
+
+template <int LPR, int RB, int NV, bool BVEC>
+hipError_t run_multi(int wpb, const GemvNParams &p, hipStream_t st) {
+  const int64_t rows = (int64_t)(64 / LPR) * RB;
+  const int64_t tasks = ((p.M + rows - 1) / rows) * p.split;
+  const int64_t nblk = (tasks + wpb - 1) / wpb;
+  if (nblk > 0x7fffffff) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL((k_gemv_multi<LPR, RB, NV, BVEC>), dim3((unsigned)nblk), dim3(64 * wpb), 0,
+                     st, p);
+  return hipGetLastError();
+}
+
+typedef hipError_t (*MultiFn)(int, const GemvNParams &, hipStream_t);
+
+template <int LPR, int RB, int NV>
+MultiFn multi_fn(bool bvec) {
+  return bvec ? run_multi<LPR, RB, NV, true> : run_multi<LPR, RB, NV, false>;
+}
+
+template <int LPR, int RB>
+MultiFn multi_nv(int nv, bool bvec) {
+  switch (nv) {
+    case 2: return multi_fn<LPR, RB, 2>(bvec);
+    case 4: return multi_fn<LPR, RB, 4>(bvec);
+    case 8: return multi_fn<LPR, RB, 8>(bvec);
+    default: return nullptr;
+  }
+}
+
+template <int LPR>
+MultiFn multi_rb(int rb, int nv, bool bvec) {
+  switch (rb) {
+    case 1: return multi_nv<LPR, 1>(nv, bvec);
+    case 2: return multi_nv<LPR, 2>(nv, bvec);
+    case 4: return multi_nv<LPR, 4>(nv, bvec);
+    default: return nullptr;
+  }
+}
+
+MultiFn find_multi(int lpr, int rb, int nv, bool bvec) {
+  switch (lpr) {
+    case 16: return multi_rb<16>(rb, nv, bvec);
+    case 32: return multi_rb<32>(rb, nv, bvec);
+    case 64: return multi_rb<64>(rb, nv, bvec);
+    default: return nullptr;
+  }
+}
+
 // Every in-range lane offset of the buffer kernels is below 2^31.
 bool gemv_buf_fits(int family, int lpr, int rb, const GemvParams &p) {
   const int64_t G = 64 / lpr;
@@ -736,6 +964,27 @@ RunFn find_gemv(int family, int lpr, int rb, bool vec) {
 }
 
 }  // namespace
+
+bool gemv_multi_has_variant(int lpr, int rb, int nv) {
+  return find_multi(lpr, rb, nv, true) != nullptr;
+}
+
+bool gemv_multi_offsets_fit(int lpr, int rb, int nv, bool bvec, const GemvNParams &p) {
+  const double lim = 2147483647.0;
+  const double a = ((double)(64 / lpr) * rb * (double)p.lda + (double)p.chunk + 4.0) * 4.0;
+  const double b = bvec ? ((double)nv * (double)p.sBn + (double)p.chunk + 4.0) * 4.0
+                        : (((double)p.chunk + 4.0) * (double)p.sBk + nv) * 4.0;
+  return a < lim && b < lim;
+}
+
+hipError_t launch_gemv_multi(int lpr, int rb, int nv, bool bvec, int wpb, const GemvNParams &p,
+                             hipStream_t st) {
+  MultiFn fn = find_multi(lpr, rb, nv, bvec);
+  if (!fn || !gemv_wpb_ok(1, wpb) || p.split < 1 || (p.wsum && p.split % wpb != 0) ||
+      !gemv_multi_offsets_fit(lpr, rb, nv, bvec, p))
+    return hipErrorInvalidValue;
+  return fn(wpb, p, st);
+}
 
 bool gemv_wpb_ok(int family, int wpb) {
   return wpb == 1 || wpb == 2 || wpb == 4 || (family == 1 && wpb == kGemvMaxWaves);

@@ -28,6 +28,11 @@ __device__ __forceinline__ f4 buf_load4(__amdgpu_buffer_rsrc_t rs, uint32_t voff
                                         int32_t soff) {
   return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, AUX));
 }
+typedef float f2 __attribute__((ext_vector_type(2)));
+template <int AUX = 0>
+__device__ __forceinline__ f2 buf_load2(__amdgpu_buffer_rsrc_t rs, uint32_t voff, int32_t soff) {
+  return __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, AUX));
+}
 template <int AUX = 0>
 __device__ __forceinline__ float buf_load1(__amdgpu_buffer_rsrc_t rs, uint32_t voff,
                                            int32_t soff) {

@@ -55,8 +55,10 @@ typedef enum {
   HML_SGEMM_FAMILY_BF16X3 = 4, /* split-precision bf16 MFMA, 2-term split  */
   HML_SGEMM_FAMILY_DENSE  = 5, /* plain FP32 MFMA dense tile kernel        */
   HML_SGEMM_FAMILY_BF16X6 = 6, /* split-precision bf16 MFMA, 3-term split  */
-  HML_SGEMM_FAMILY_DOTB   = 7  /* MFMA f32, K-contiguous A, B shared by the
+  HML_SGEMM_FAMILY_DOTB   = 7, /* MFMA f32, K-contiguous A, B shared by the
                                   workgroup through LDS                   */
+  HML_SGEMM_FAMILY_GEMVN  = 8  /* N in {2, 4, 8}, K-contiguous A: the GEMV
+                                  DOT kernels with N accumulators per row */
 } hmlSgemmFamily;
 
 /* Skinny variants (DOT, AXPY): a wave computes rt row tiles (16 rows DOT,
@@ -69,7 +71,9 @@ typedef enum {
  * reduction pass; rt in {1, 2}, ct in {1, 2, 4}.  AXPY: like DOT, the wpb
  * (1, 2, 4, or 8 when rt * ct < 8) waves of a workgroup take consecutive
  * chunks of one tile and sum them in LDS; rt in {1, 2}, ct in {1, 2, 4, 8},
- * not 2 x 8. */
+ * not 2 x 8.  GEMVN: rt = lanes per row (16, 32, 64), ct = rows per lane
+ * group (1, 2, 4), ksplit = chunks of K, wpb (1, 2, 4, 8) as for the DOT
+ * SGEMV variants. */
 typedef struct {
   int32_t family; /* hmlSgemmFamily                                          */
   int32_t rt;     /* row tiles per wave                                       */

@@ -69,7 +69,9 @@ def test_integer_exact_layouts(hml, shape, layout):
         np.testing.assert_array_equal(C, exact_product(A, B, alpha, beta, C0))
 
 
-CAND_SHAPES = [(4096, 16, 2048, "row", False, False),
+CAND_SHAPES = [(3000, 4, 700, "row", False, False),
+               (2049, 8, 1000, "row", False, True),
+               (4096, 16, 2048, "row", False, False),
                (8192, 64, 512, "row", False, False),
                (3000, 33, 700, "row", False, False),
                (4096, 16, 2048, "row", True, False),
@@ -154,6 +156,42 @@ def test_axpy_variants_exact(hml, tb):
                     dC = hml.DeviceBuffer.from_numpy(np.ascontiguousarray(C0))
                     hml.sgemm_ex(M, N, K, dA, dB, dC, 1.0, 1.0, order="row", transA=True,
                                  transB=tb, lda=lda, ldb=ldb, ldc=N, variant=v)
+                    hml.synchronize()
+                    np.testing.assert_array_equal(dC.to_numpy((M, N)), R, err_msg=repr(v))
+                    dC.free()
+        dA.free()
+        dB.free()
+
+
+GEMVN_SHAPES = [(1000, 2, 700, 0), (4097, 4, 1000, 3), (33, 8, 4100, 0), (300, 8, 63, 1),
+                (64, 2, 5, 0), (2049, 4, 256, 0)]
+
+
+@pytest.mark.parametrize("tb", [False, True], ids=["BN", "BT"])
+def test_gemvn_variants_exact(hml, tb):
+    """The multi-vector GEMV family (8) for 2, 4 and 8 columns: every lanes-
+    per-row / rows-per-group instantiation, no split, splits summed inside
+    the workgroup and across workgroups, B N-contiguous and K-contiguous,
+    ragged M and K, K below one step, padded lda (odd, breaking 16-byte
+    alignment): bit-exact on integer data."""
+    rng = np.random.default_rng(29)
+    for (M, N, K, pad) in GEMVN_SHAPES:
+        A = int_matrix(rng, (M, K))
+        B = int_matrix(rng, (K, N))
+        C0 = int_matrix(rng, (M, N))
+        R = exact_product(A, B, 1.0, 1.0, C0)
+        lda = K + pad
+        dA = hml.DeviceBuffer.from_numpy(padded(np.ascontiguousarray(A), lda))
+        Bs, ldb = store(B, "row", tb)
+        dB = hml.DeviceBuffer.from_numpy(Bs)
+        for lpr in (16, 32, 64):
+            for rb in (1, 2, 4):
+                for ks, wpb in ((1, 2), (3, 2), (8, 8), (16, 4)):
+                    v = hml.SgemmVariant(8, lpr, rb, ks, wpb)
+                    print("case", M, N, K, pad, "BT" if tb else "BN", v, flush=True)
+                    dC = hml.DeviceBuffer.from_numpy(np.ascontiguousarray(C0))
+                    hml.sgemm_ex(M, N, K, dA, dB, dC, 1.0, 1.0, order="row", transB=tb,
+                                 lda=lda, ldb=ldb, ldc=N, variant=v)
                     hml.synchronize()
                     np.testing.assert_array_equal(dC.to_numpy((M, N)), R, err_msg=repr(v))
                     dC.free()

@@ -140,3 +140,17 @@ def test_gemv_axpy_small_split_target():
     ncb = lambda n, lpr: -(-n // (4 * lpr))
     assert small.family == 2 and ncb(2049, small.lpr) * small.split >= 1000
     assert big.family == 2 and ncb(16384, big.lpr) * big.split <= 600
+
+
+def test_heuristic_two_columns_multi_vector():
+    """Two columns with A contiguous along K go to the multi-vector GEMV
+    family (8) with the SGEMV geometry of the same rows; 4 and 8 columns
+    stay on the matrix-core tiles, and the AXPY form is unaffected."""
+    sel = lambda M, N, K, **kw: hml.sgemm_select(M, N, K, **kw)[0]
+    v = sel(8192, 2, 8192)
+    g = hml.sgemv_select(8192, 8192)[0]
+    assert (v.family, v.rt, v.ct, v.ksplit, v.wpb) == (8, g.lpr, min(g.rb, 4), g.split, g.wpb)
+    assert sel(8192, 2, 8192, transB=True).family == 8
+    assert sel(8192, 4, 8192).family == 1
+    assert sel(8192, 8, 8192).family == 1
+    assert sel(8192, 2, 8192, transA=True).family == 2
