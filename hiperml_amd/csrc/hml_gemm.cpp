@@ -336,6 +336,19 @@ hmlStatus run_gemv_family(const GemmCanon &g, const GemmKey &k,
                           const hmlSgemmVariant &v, float alpha, float beta,
                           int read_c, void *ws, hipStream_t st);
 
+// Strides too large for the 31-bit buffer offsets of the skinny and
+// multi-vector kernels: plain FP32 dense tiles instead (they address with
+// 64-bit pointers through the split pass).
+hmlStatus run_dense_fallback(const GemmCanon &g, const GemmKey &k, float alpha, float beta,
+                             int read_c, hipStream_t st) {
+  const size_t dn = dense_workspace(0, k.M, k.N, k.K);
+  void *dws = nullptr;
+  hmlStatus s = library_workspace(dn, &dws);
+  if (s != HML_STATUS_SUCCESS) return s;
+  return run_dense(0, g.A, g.sAm, g.sAk, g.B, g.sBk, g.sBn, g.C, g.sCm, g.sCn, g.M, g.N, g.K,
+                   alpha, beta, read_c, dws, dn, st);
+}
+
 hmlStatus run_sgemm(const GemmCanon &g, const GemmKey &k,
                     const hmlSgemmVariant &v, float alpha, float beta,
                     int read_c, void *ws, size_t ws_bytes, hipStream_t st) {
@@ -363,7 +376,7 @@ hmlStatus run_sgemm(const GemmCanon &g, const GemmKey &k,
     p.chunk = sp.chunk; p.split = sp.split; p.wsum = sp.wsum;
     const bool bvec = g.sBk == 1 || g.K == 1;
     if (!gemv_multi_offsets_fit(v.rt, v.ct, (int)g.N, bvec, p))
-      return fail(HML_STATUS_NO_KERNEL, "strides too large for the multi-vector GEMV kernel");
+      return run_dense_fallback(g, k, alpha, beta, read_c, st);
     hipError_t e = launch_gemv_multi(v.rt, v.ct, (int)g.N, bvec, v.wpb, p, st);
     if (e != hipSuccess) return from_hip(e, "multi-vector gemv launch");
     if (slots > 1) {
@@ -378,16 +391,7 @@ hmlStatus run_sgemm(const GemmCanon &g, const GemmKey &k,
                      read_c, ws, ws ? need : 0, st);
 
   const SkinnyPlan pl = plan_skinny(k, v);
-  if (!skinny_offsets_fit(g, v, pl)) {
-    // strides too large for 31-bit buffer offsets: dense tiles instead
-    // (they address with 64-bit pointers through the split pass)
-    const size_t dn = dense_workspace(0, k.M, k.N, k.K);
-    void *dws = nullptr;
-    hmlStatus s = library_workspace(dn, &dws);
-    if (s != HML_STATUS_SUCCESS) return s;
-    return run_dense(0, g.A, g.sAm, g.sAk, g.B, g.sBk, g.sBn, g.C, g.sCm, g.sCn,
-                     g.M, g.N, g.K, alpha, beta, read_c, dws, dn, st);
-  }
+  if (!skinny_offsets_fit(g, v, pl)) return run_dense_fallback(g, k, alpha, beta, read_c, st);
   hipError_t e;
   if (k.form == kFormDot) {
     DotParams p{};
