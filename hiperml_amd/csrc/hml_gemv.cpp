@@ -98,6 +98,13 @@ void heuristic_sgemv(const GemvKey &k, hmlSgemvVariant *v) {
       split = std::min<int64_t>(split, std::max<int64_t>(1, k.reduced / (4 * v->lpr * 4)));
     }
     v->split = (int)std::max<int64_t>(1, std::min<int64_t>(split, 1024));
+    // a split sums inside workgroups of up to 8 waves: no partials and no
+    // reduction launch up to 8 chunks (16384^2: 0.151 ms, within noise of
+    // the best split variant, and the call is then one kernel)
+    if (v->split > 1) {
+      v->wpb = 2;
+      while (v->wpb * 2 <= std::min(v->split, kGemvMaxWaves)) v->wpb *= 2;
+    }
   } else {
     // Column chunks of at most 128 floats (32 lanes); the rows split so
     // that about 512 workgroups of 4 waves (2048 waves) run -- 1024 below

@@ -713,6 +713,7 @@ __global__ __launch_bounds__(64 * kGemvMaxWaves) void k_gemv_multi(GemvNParams p
   for (int i = 0; i < RB; ++i)
 #pragma unroll
     for (int n = 0; n < NV; ++n) acc[i][n] = 0.f;
+  const bool dense2 = !BVEC && NV == 2 && p.sBk == 2 && p.sBn == 1;
 
   f4 ba[PS][RB];
   float bx[PS][4][NV];
@@ -732,6 +733,17 @@ __global__ __launch_bounds__(64 * kGemvMaxWaves) void k_gemv_multi(GemvNParams p
       }
     } else {
       const int32_t sob = (int32_t)(s * STEP * p.sBk * 4);
+      if (NV == 2 && dense2) {
+        // rows of 2 floats back to back: the 4 rows are 32 contiguous bytes
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const f4 t = buf_load4<0>(rs_b, boff[0] + 16 * h, sob);
+          bx[j][2 * h][0] = t[0];
+          bx[j][2 * h][1] = t[1];
+          bx[j][2 * h + 1][0] = t[2];
+          bx[j][2 * h + 1][1] = t[3];
+        }
+      } else {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         if (NV == 2) {
@@ -746,6 +758,7 @@ __global__ __launch_bounds__(64 * kGemvMaxWaves) void k_gemv_multi(GemvNParams p
             for (int n = 0; n < 4; ++n) bx[j][e][4 * h + n] = t[n];
           }
         }
+      }
       }
     }
   };

@@ -164,7 +164,7 @@ def test_axpy_variants_exact(hml, tb):
 
 
 GEMVN_SHAPES = [(1000, 2, 700, 0), (4097, 4, 1000, 3), (33, 8, 4100, 0), (300, 8, 63, 1),
-                (64, 2, 5, 0), (2049, 4, 256, 0)]
+                (64, 2, 5, 0), (2049, 4, 256, 0), (777, 2, 1030, 1)]
 
 
 @pytest.mark.parametrize("tb", [False, True], ids=["BN", "BT"])
@@ -173,7 +173,8 @@ def test_gemvn_variants_exact(hml, tb):
     per-row / rows-per-group instantiation, no split, splits summed inside
     the workgroup and across workgroups, B N-contiguous and K-contiguous,
     ragged M and K, K below one step, padded lda (odd, breaking 16-byte
-    alignment): bit-exact on integer data."""
+    alignment) and, with it, B rows not back to back (NaN in the gap):
+    bit-exact on integer data."""
     rng = np.random.default_rng(29)
     for (M, N, K, pad) in GEMVN_SHAPES:
         A = int_matrix(rng, (M, K))
@@ -183,12 +184,15 @@ def test_gemvn_variants_exact(hml, tb):
         lda = K + pad
         dA = hml.DeviceBuffer.from_numpy(padded(np.ascontiguousarray(A), lda))
         Bs, ldb = store(B, "row", tb)
+        if pad % 2:  # B rows with a gap (not back to back)
+            ldb += 1
+            Bs = padded(Bs, ldb)
         dB = hml.DeviceBuffer.from_numpy(Bs)
         for lpr in (16, 32, 64):
             for rb in (1, 2, 4):
                 for ks, wpb in ((1, 2), (3, 2), (8, 8), (16, 4)):
                     v = hml.SgemmVariant(8, lpr, rb, ks, wpb)
-                    print("case", M, N, K, pad, "BT" if tb else "BN", v, flush=True)
+                    print("case", M, N, K, pad, ldb, "BT" if tb else "BN", v, flush=True)
                     dC = hml.DeviceBuffer.from_numpy(np.ascontiguousarray(C0))
                     hml.sgemm_ex(M, N, K, dA, dB, dC, 1.0, 1.0, order="row", transB=tb,
                                  lda=lda, ldb=ldb, ldc=N, variant=v)
