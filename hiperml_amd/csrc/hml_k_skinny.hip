@@ -194,9 +194,7 @@ __device__ __forceinline__ void dot_tile_lds(const float *__restrict__ A, int64_
     if constexpr (DMA) {
 #pragma unroll
       for (int q = 0; q < NQ; ++q)
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(
-            rs, (__attribute__((address_space(3))) void *)(T + j * TF + q * 256), 16, aoff[q],
-            so, 0, kBufNT);
+        buf_load_lds16<kBufNT>(rs, T + j * TF + q * 256, aoff[q], so);
     } else {
 #pragma unroll
       for (int q = 0; q < NQ; ++q) a[j][q] = buf_load4<kBufNT>(rs, aoff[q], so);
@@ -406,7 +404,7 @@ __global__ __launch_bounds__(64 * dot_max_waves(RT, CT)) void k_skinny_dot(DotPa
 // ---------------------------------------------------------------------------
 // Persistent grouped SGEMM: hundreds of independent skinny problems in one
 // graph launch.  The host sorts the problems into tile classes by N (2 x 1,
-// 2 x 2 or 1 x 4 tiles of 16-row x 16-column blocks), builds one descriptor
+// 2 x 2 or 2 x 4 tiles of 16-row x 16-column blocks), builds one descriptor
 // per problem and, per class, a list of (problem, row block) tiles sorted by
 // decreasing cost; one persistent kernel per class runs its list: wave w
 // takes tiles w, w + W, w + 2W, ... (W = waves launched), a static
@@ -414,13 +412,15 @@ __global__ __launch_bounds__(64 * dot_max_waves(RT, CT)) void k_skinny_dot(DotPa
 // be captured in a graph and replayed.  Each tile runs the DOT tile body
 // above (B read in place, whole K, no split) over its problem's column
 // tiles.  One kernel per class keeps each at its own register budget (a
-// single kernel dispatching on the class would pay for the largest).
+// single kernel dispatching on the class would pay for the largest); the
+// 4-column class stages A through LDS DMA, which is what lets its 2 x 4
+// tile fit 2 waves per SIMD without spilling.
 // ---------------------------------------------------------------------------
 template <int RT, int CT>
 __global__ __launch_bounds__(256, 2) void k_grouped_dot(const GroupProblem *probs,
                                                      const GroupTile *tiles,
                                                      int64_t ntiles) {
-  __shared__ f4 gtile[4][dot_tile_floats(RT) / 4];
+  __shared__ f4 gtile[4][(dot_dma(RT, CT) ? dot_stages(RT, CT) : 1) * dot_tile_floats(RT) / 4];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int r = lane & 15, g = lane >> 4;
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(256, 2) void k_grouped_dot(const GroupProblem *prob
     const int64_t nct = (gp.N + 15) / 16;
     for (int ct0 = 0; ct0 < nct; ct0 += CT) {
       f4 acc[RT][CT];
-      dot_tile_lds<RT, CT, false, false>(gp.A, gp.lda, gp.B, gp.ldb, 1, gp.M, gp.N, row0,
+      dot_tile_lds<RT, CT, false, dot_dma(RT, CT)>(gp.A, gp.lda, gp.B, gp.ldb, 1, gp.M, gp.N, row0,
                                   ct0, 0, gp.K, T, acc);
 #pragma unroll
       for (int tt = 0; tt < RT; ++tt)
@@ -1195,14 +1195,14 @@ const void *grouped_fn(int cls) {
   switch (cls) {
     case 0: return (const void *)k_grouped_dot<2, 1>;
     case 1: return (const void *)k_grouped_dot<2, 2>;
-    case 2: return (const void *)k_grouped_dot<1, 4>;
+    case 2: return (const void *)k_grouped_dot<2, 4>;
     default: return nullptr;
   }
 }
 }  // namespace
 
 void grouped_class_shape(int cls, int *rt, int *ct) {
-  static const int kRt[kGroupClasses] = {2, 2, 1}, kCt[kGroupClasses] = {1, 2, 4};
+  static const int kRt[kGroupClasses] = {2, 2, 2}, kCt[kGroupClasses] = {1, 2, 4};
   *rt = kRt[cls];
   *ct = kCt[cls];
 }
@@ -1222,7 +1222,7 @@ hipError_t launch_grouped(int cls, const GroupProblem *probs, const GroupTile *t
   switch (cls) {
     case 0: hipLaunchKernelGGL((k_grouped_dot<2, 1>), dim3((unsigned)nblocks), dim3(256), 0, st, probs, tiles, ntiles); break;
     case 1: hipLaunchKernelGGL((k_grouped_dot<2, 2>), dim3((unsigned)nblocks), dim3(256), 0, st, probs, tiles, ntiles); break;
-    case 2: hipLaunchKernelGGL((k_grouped_dot<1, 4>), dim3((unsigned)nblocks), dim3(256), 0, st, probs, tiles, ntiles); break;
+    case 2: hipLaunchKernelGGL((k_grouped_dot<2, 4>), dim3((unsigned)nblocks), dim3(256), 0, st, probs, tiles, ntiles); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

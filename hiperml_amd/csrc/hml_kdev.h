@@ -28,6 +28,18 @@ __device__ __forceinline__ f4 buf_load4(__amdgpu_buffer_rsrc_t rs, uint32_t voff
                                         int32_t soff) {
   return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, AUX));
 }
+// 16-byte buffer load straight into LDS: lane l's 16 bytes land at
+// lds + 16 l (buffer_load_dwordx4 ... lds; the LDS base goes through M0).
+// The builtin exists only for the device target, so the host pass of a
+// kernel that instantiates this sees an empty body.
+template <int AUX>
+__device__ __forceinline__ void buf_load_lds16(__amdgpu_buffer_rsrc_t rs, float *lds,
+                                               uint32_t voff, int32_t soff) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void *)lds, 16,
+                                           voff, soff, 0, AUX);
+#endif
+}
 typedef float f2 __attribute__((ext_vector_type(2)));
 template <int AUX = 0>
 __device__ __forceinline__ f2 buf_load2(__amdgpu_buffer_rsrc_t rs, uint32_t voff, int32_t soff) {

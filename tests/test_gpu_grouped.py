@@ -37,9 +37,8 @@ def test_grouped_exact_direct(hml):
     probs, host = make_problems(hml, rng, 40)
     plan = hml.GroupedPlan(probs, 1.0, 0.0)
     info = plan.info()
-    # tile classes by N: 32-row tiles for N <= 32, 16-row tiles above
-    assert info["tiles"] == sum((p[0] + 31) // 32 if p[1] <= 32 else (p[0] + 15) // 16
-                                for p in probs)
+    # every tile class (by N: 16, 32 or 64 columns per pass) has 32-row tiles
+    assert info["tiles"] == sum((p[0] + 31) // 32 for p in probs)
     plan.run(graph=False)
     hml.synchronize()
     for (M, N, K, dA, dB, dC), (A, B, C) in zip(probs, host):
