@@ -168,6 +168,14 @@ bool selector_variant(const GemmKey &k, int cls, hmlSgemmVariant *v) {
     heuristic_sgemm(g, v);
     return v->family == HML_SGEMM_FAMILY_GEMV && sgemm_variant_runnable(k, *v);
   }
+  if (cls == 86) {
+    if (k.form != kFormDot || !(k.N == 2 || k.N == 4 || k.N == 8)) return false;
+    GemvKey gk{HML_SGEMV_FAMILY_DOT, k.M, k.K, k.beta0};
+    hmlSgemvVariant gv;
+    heuristic_sgemv(gk, &gv);
+    *v = hmlSgemmVariant{HML_SGEMM_FAMILY_GEMVN, gv.lpr, std::min(gv.rb, 4), gv.split, gv.wpb};
+    return sgemm_variant_runnable(k, *v);
+  }
   const int wpb_i = cls & 1;
   const int ks_i = (cls >> 1) % 7;
   const int rest = (cls >> 1) / 7;  // rt_i * 2 + ctm

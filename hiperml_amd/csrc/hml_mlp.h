@@ -30,9 +30,11 @@ hipError_t mlp_gather_rows(const float *src, const int32_t *idx, float *dst,
 // Selector class space.  Classes 0..83: skinny MFMA variants (rt index 0..2
 // -> 1/2/4 row tiles, column mode 0..1 -> full / half the column tiles that
 // cover N (DOT: at most 4), ksplit 2^0..2^6, wpb 4 or 8); 84: dense FP32
-// tiles; 85: the GEMV kernels (heuristic parameters).  The family of a
-// skinny class follows the problem's form (dot or axpy).
+// tiles; 85: the GEMV kernels (heuristic parameters); 86: the multi-vector
+// GEMV kernels for N = 2, 4, 8 with A contiguous along K (the SGEMV
+// heuristic's parameters for the same rows).  The family of a skinny class
+// follows the problem's form (dot or axpy).
 constexpr int kSelFeatures = 6;
-constexpr int kSelClasses = 86;
+constexpr int kSelClasses = 87;
 
 }  // namespace hml

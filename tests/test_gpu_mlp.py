@@ -91,7 +91,7 @@ def test_selector_install_and_fallback(hml, tmp_path):
                   for m, n, k in zip(2 ** rng.integers(12, 20, 400),
                                      2 ** rng.integers(0, 6, 400),
                                      2 ** rng.integers(8, 14, 400))])
-    T = np.zeros((400, 86), np.float32)
+    T = np.zeros((400, hml.lib.hmlSelectorClassCount()), np.float32)
     T[:, cls] = 1
     net = hml.Mlp(seed=2)
     net.train(X, T, np.ones_like(T), epochs=40, batch=100, lr=3e-3)
