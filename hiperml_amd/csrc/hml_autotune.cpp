@@ -124,7 +124,6 @@ std::vector<hmlSgemvVariant> sgemv_candidates(const GemvKey &k) {
     const int64_t red = k.reduced;
     for (int rb : k.family == HML_SGEMV_FAMILY_DOT ? std::vector<int>{1, 2, 4, 8} : std::vector<int>{1}) {
       for (int wpb : {2, 4, 8}) {
-        if (wpb == 8 && k.family != HML_SGEMV_FAMILY_DOT) continue;
         const int64_t cap = k.family == HML_SGEMV_FAMILY_DOT
                                 ? std::max<int64_t>(1, red / (4 * lpr * 4))
                                 : std::max<int64_t>(1, red / 64);

@@ -106,16 +106,18 @@ void heuristic_sgemv(const GemvKey &k, hmlSgemvVariant *v) {
       while (v->wpb * 2 <= std::min(v->split, kGemvMaxWaves)) v->wpb *= 2;
     }
   } else {
-    // Column chunks of at most 128 floats (32 lanes); the rows split so
-    // that about 512 workgroups of 4 waves (2048 waves) run -- 1024 below
-    // 64 MB of A, where each wave's short row range is latency-bound
-    // (4099 x 2049: 18.5 -> 12 us) -- each split at least 64 rows long, at
-    // most 512 splits.
+    // Column chunks of at most 128 floats (32 lanes), 8-wave workgroups
+    // (their waves' sums meet in LDS); the rows split so that about 256
+    // workgroups (2048 waves) run -- 1024 below 64 MB of A, where each
+    // wave's short row range is latency-bound (4099 x 2049: 18.5 -> 12 us)
+    // -- each split at least 64 rows long, at most 512 splits.  Measured:
+    // 16384^2 A^T x 0.163 -> 0.153 ms and 2^20 x 64 0.048 -> 0.044 ms
+    // against 4-wave workgroups on twice the splits.
     v->lpr = std::min(32, lpr_for(k.outputs));
     v->rb = 1;
-    v->wpb = 4;
+    v->wpb = 8;
     const int64_t ncb = ceil_div(k.outputs, 4 * v->lpr);
-    const int64_t target_wg = (double)k.outputs * (double)k.reduced * 4.0 < 67108864.0 ? 1024 : 512;
+    const int64_t target_wg = (double)k.outputs * (double)k.reduced * 4.0 < 67108864.0 ? 1024 : 256;
     int64_t split = ceil_div(target_wg, ncb);
     split = std::min<int64_t>(split, std::max<int64_t>(1, k.reduced / 64));
     v->split = (int)std::max<int64_t>(1, std::min<int64_t>(split, 512));

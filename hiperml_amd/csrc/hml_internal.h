@@ -136,7 +136,7 @@ hipError_t launch_scale(float *C, int64_t M, int64_t N, int64_t sCm,
 hipError_t launch_gemv(int family, int lpr, int rb, bool vec, int wpb,
                        const GemvParams &p, hipStream_t st);
 bool gemv_has_variant(int family, int lpr, int rb);
-// Workgroup sizes (waves) the family's kernels take: 1, 2, 4, and 8 for DOT.
+// Workgroup sizes (waves) the GEMV kernels take: 1, 2, 4 or 8.
 bool gemv_wpb_ok(int family, int wpb);
 // Resolves a GEMV variant's split of `reduced` into p->chunk, p->split and
 // p->wsum; returns the partial slices the launch writes (1: none, y

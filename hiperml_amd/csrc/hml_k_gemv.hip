@@ -160,10 +160,10 @@ __global__ __launch_bounds__(64 * kGemvMaxWaves) void k_gemv_dot(GemvParams p) {
 }
 
 template <int LPR, bool VEC>
-__global__ __launch_bounds__(256) void k_gemv_axpy(GemvParams p) {
+__global__ __launch_bounds__(64 * kGemvMaxWaves) void k_gemv_axpy(GemvParams p) {
   constexpr int G = 64 / LPR;
   constexpr int CW = 4 * LPR;
-  __shared__ f4 red[4][LPR];
+  __shared__ f4 red[kGemvMaxWaves][LPR];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
@@ -329,11 +329,11 @@ __global__ __launch_bounds__(64 * kGemvMaxWaves) void k_gemv_dot_stream(GemvPara
 }
 
 template <int LPR>
-__global__ __launch_bounds__(256) void k_gemv_axpy_stream(GemvParams p) {
+__global__ __launch_bounds__(64 * kGemvMaxWaves) void k_gemv_axpy_stream(GemvParams p) {
   constexpr int G = 64 / LPR;
   constexpr int CW = 4 * LPR;
   constexpr int P = 4;
-  __shared__ f4 red[4][LPR];
+  __shared__ f4 red[kGemvMaxWaves][LPR];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
@@ -540,11 +540,11 @@ __global__ __launch_bounds__(64 * kGemvMaxWaves) void k_gemv_dot_buf(GemvParams 
 }
 
 template <int LPR>
-__global__ __launch_bounds__(256) void k_gemv_axpy_buf(GemvParams p) {
+__global__ __launch_bounds__(64 * kGemvMaxWaves) void k_gemv_axpy_buf(GemvParams p) {
   constexpr int G = 64 / LPR;
   constexpr int CW = 4 * LPR;
   constexpr int P = 4;
-  __shared__ f4 red[4][LPR];
+  __shared__ f4 red[kGemvMaxWaves][LPR];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wpb = blockDim.x >> 6;
@@ -999,8 +999,8 @@ hipError_t launch_gemv_multi(int lpr, int rb, int nv, bool bvec, int wpb, const 
   return fn(wpb, p, st);
 }
 
-bool gemv_wpb_ok(int family, int wpb) {
-  return wpb == 1 || wpb == 2 || wpb == 4 || (family == 1 && wpb == kGemvMaxWaves);
+bool gemv_wpb_ok(int, int wpb) {
+  return wpb == 1 || wpb == 2 || wpb == 4 || wpb == kGemvMaxWaves;
 }
 
 bool gemv_has_variant(int family, int lpr, int rb) {
