@@ -91,8 +91,10 @@ typedef enum {
 /* SGEMV variants.  DOT with split > 1 and wpb > 1: the wpb waves of a
  * workgroup take consecutive chunks of the same rows and sum them in LDS
  * (split is rounded up to a multiple of wpb), so only split / wpb > 1
- * leaves partials for a reduction pass.  wpb is 1, 2, 4 or 8 (DOT), 1, 2
- * or 4 (AXPY). */
+ * leaves partials for a reduction pass.  AXPY: the wpb waves of a
+ * workgroup take interleaved rows of one column chunk and add their sums in
+ * LDS; split chunks of rows are summed by a reduction pass.  wpb is 1, 2, 4
+ * or 8. */
 typedef struct {
   int32_t family; /* hmlSgemvFamily                                      */
   int32_t lpr;    /* lanes per row (DOT) / lanes per column chunk (AXPY)  */
