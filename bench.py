@@ -192,8 +192,8 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": None,
-                "kernel": "k_gemv_dot_stream<64,2>: one SGEMV A x call (a single launch: the "
-                          "8-way row split sums inside 8-wave workgroups), 16384 x 16384",
+                "kernel": "k_gemv_dot_stream<64,2> + its split reduction: one SGEMV A x call, "
+                          "16384 x 16384",
                 "bytes_per_launch": d[4], "avg_launch_ms": round(d_ms, 5)}
     traffic_file = os.path.join(here, "profiles", "traffic_gemv_dot_16384.json")
     if os.path.exists(traffic_file):

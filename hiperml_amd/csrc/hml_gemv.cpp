@@ -88,7 +88,12 @@ void heuristic_sgemv(const GemvKey &k, hmlSgemvVariant *v) {
     // Split the rows into enough pieces for about 64K waves when A is at
     // least 512 MB, 4K below that (a split adds partials and a reduction
     // pass, which costs more than it hides on mid-size problems), each
-    // piece at least 4 loads per lane long.
+    // piece at least 4 loads per lane long.  The two waves of a workgroup
+    // add their chunks in LDS.  Measured across 16384^2, 32768^2,
+    // 65536 x 8192 and 8192 x 65536: this two-row, two-wave geometry is the
+    // best or within 3% of it; workgroups of 8 waves covering the whole
+    // split were up to 7% slower, single-row waves on a 256K-wave split 3%
+    // faster on 16384^2 only and up to 7% slower on the others.
     const int64_t rows_per_wave = (int64_t)G * v->rb;
     const int64_t waves = ceil_div(k.outputs, rows_per_wave);
     const int64_t target = (double)k.outputs * (double)k.reduced * 4.0 >= 536870912.0 ? 65536 : 4096;
@@ -98,13 +103,6 @@ void heuristic_sgemv(const GemvKey &k, hmlSgemvVariant *v) {
       split = std::min<int64_t>(split, std::max<int64_t>(1, k.reduced / (4 * v->lpr * 4)));
     }
     v->split = (int)std::max<int64_t>(1, std::min<int64_t>(split, 1024));
-    // a split sums inside workgroups of up to 8 waves: no partials and no
-    // reduction launch up to 8 chunks (16384^2: 0.151 ms, within noise of
-    // the best split variant, and the call is then one kernel)
-    if (v->split > 1) {
-      v->wpb = 2;
-      while (v->wpb * 2 <= std::min(v->split, kGemvMaxWaves)) v->wpb *= 2;
-    }
   } else {
     // Column chunks of at most 128 floats (32 lanes), 8-wave workgroups
     // (their waves' sums meet in LDS); the rows split so that about 256
