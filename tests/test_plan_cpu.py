@@ -154,3 +154,28 @@ def test_heuristic_two_columns_multi_vector():
     assert sel(8192, 4, 8192).family == 1
     assert sel(8192, 8, 8192).family == 1
     assert sel(8192, 2, 8192, transA=True).family == 2
+
+
+@pytest.mark.parametrize("M,N,K,ks,wpb,slices", [
+    (8192, 2, 8192, 1, 2, 0),     # no split
+    (8192, 4, 8192, 8, 2, 4),     # 8 chunks, pairs summed in LDS: 4 slices
+    (8192, 8, 8192, 3, 2, 2),     # 3 chunks rounded up to 4: 2 slices
+    (1000, 2, 700, 8, 8, 0),      # one 8-wave workgroup covers the split
+])
+def test_gemvn_workspace(M, N, K, ks, wpb, slices):
+    """Multi-vector GEMV family (8): partial slices of M x N floats follow
+    the SGEMV DOT split rules (lanes per row 64 here)."""
+    got = ws(M, N, K, (8, 64, 2, ks, wpb))
+    assert got == (0 if slices <= 1 else (slices * M * N * 4 + 255) // 256 * 256)
+
+
+def test_gemvn_runnable_and_selector_class():
+    """Family 8 runs only for 2, 4 or 8 columns with A contiguous along K;
+    selector class 86 is that family with the SGEMV heuristic's geometry."""
+    assert hml.selector_variant(8192, 2, 8192, 86).astuple()[0] == 8
+    assert hml.selector_variant(8192, 16, 8192, 86) is None          # not 2/4/8 columns
+    assert hml.selector_variant(8192, 2, 8192, 86, transA=True) is None  # AXPY form
+    m = hml.selector_mask(8192, 4, 8192)
+    assert m[86] == 1
+    v = hml.selector_variant(8192, 4, 8192, 86)
+    assert hml.selector_class_of(8192, 4, 8192, v) == 86
