@@ -6,7 +6,9 @@
 // times, the median decides, the winner per shape key becomes the config
 // rule, a launch failure invalidates only that candidate.  What changes:
 //  * every candidate is VALIDATED before it is timed (the reference times
-//    only, which let its broken BasicN reduction win): the
+//    only, which let its BasicN reduction win although it drops partial
+//    sums for blockStops 6/7/9/10, include/hml_sgemv_kernel_template.h:67-100,
+//    src/hml_sgemv_kernel.cu:491-533): the
 //    shape's reference output is itself checked against the FP64 kernel on
 //    three row windows, then each candidate's full output is compared with
 //    it on the device;
