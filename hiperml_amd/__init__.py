@@ -204,7 +204,7 @@ def sgemv_ex(M, N, A, x, y, alpha=1.0, beta=0.0, order="row", trans=False,
     check(st)
 
 
-PRECISION = {"fp32": 0, "bf16x3": 1, "bf16x6": 2}
+PRECISION = {"fp32": 0, "bf16x3": 1, "bf16x6": 2, "f16x3": 3}
 
 
 KMEANS_SEPARATE_ROUNDING = 0x1

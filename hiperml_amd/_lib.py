@@ -196,6 +196,8 @@ _SIGS = {
     "hmlEventElapsedMs": (c_i32, [ctypes.POINTER(ctypes.c_float), c_void_p,
                                   c_void_p]),
     "hmlFlushCaches": (c_i32, [c_void_p]),
+    "hmlReadProbe": (c_i32, [c_void_p, c_size, c_void_p, c_void_p]),
+    "hmlAutotuneStats": (c_i32, [ctypes.c_char_p, c_i32, ctypes.c_char_p]),
     "hmlStatusString": (ctypes.c_char_p, [c_i32]),
     "hmlLastError": (ctypes.c_char_p, []),
 }

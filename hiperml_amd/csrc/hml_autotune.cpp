@@ -17,7 +17,11 @@
 //    does not flatter small shapes; inputs are uniform[-1,1] (the reference
 //    times on memset byte patterns 11/22/33);
 //  * every (shape, variant, median) is appended to a CSV timing database,
-//    the training set of the learned selector;
+//    the training set of the learned selector; the database is also the
+//    checkpoint (the reference's is the .cfg it writes at the end,
+//    src/hml_sgemm_kernel.cu:557-602): a candidate already recorded for a
+//    shape is not measured again, so an interrupted sweep resumes where it
+//    stopped and a finished shape costs no GPU work;
 //  * a fake timer can replace the GPU so the search/selection logic is
 //    testable on a CPU.
 #include "hml_autotune.h"
@@ -28,7 +32,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <set>
+#include <string>
 #include <vector>
 
 #include "hml_config.h"
@@ -204,6 +211,80 @@ FILE *db_open(const char *path) {
   return f;
 }
 
+// ---- timing database index (resume) ----------------------------------------
+// Rows already in a database, by shape: the file is read incrementally (only
+// what was appended since the last call), so a sweep over thousands of
+// shapes does not re-read it per shape.
+struct DbRow {
+  int32_t var[5];
+  double ms;
+  int valid;
+};
+struct DbIndex {
+  std::string path;
+  long offset = 0;
+  std::map<std::string, std::vector<DbRow>> rows;
+};
+std::mutex g_db_mutex;
+DbIndex g_db;
+
+std::string db_key(int kind, const int64_t *dims, const int32_t *ctx) {
+  char b[160];
+  snprintf(b, sizeof b, "%d,%lld,%lld,%lld,%d,%d,%d,%d", kind, (long long)dims[0],
+           (long long)dims[1], (long long)dims[2], ctx[0], ctx[1], ctx[2], ctx[3]);
+  return b;
+}
+
+// Rows recorded for one shape (empty without a database).
+std::vector<DbRow> db_prior(const char *path, int kind, const int64_t *dims,
+                            const int32_t *ctx) {
+  std::vector<DbRow> out;
+  if (!path) return out;
+  std::lock_guard<std::mutex> lk(g_db_mutex);
+  if (g_db.path != path) {
+    g_db = DbIndex();
+    g_db.path = path;
+  }
+  FILE *f = fopen(path, "r");
+  if (f) {
+    fseek(f, 0, SEEK_END);
+    const long end = ftell(f);
+    if (end < g_db.offset) {  // the file was replaced: start over
+      g_db.rows.clear();
+      g_db.offset = 0;
+    }
+    fseek(f, g_db.offset, SEEK_SET);
+    char line[512];
+    while (fgets(line, sizeof line, f)) {
+      const size_t len = strlen(line);
+      if (len == 0 || line[len - 1] != '\n') break;  // partial last line
+      g_db.offset += (long)len;
+      int k, c[4], form, v[5], valid;
+      long long d[3];
+      double ms;
+      if (sscanf(line, "%d,%lld,%lld,%lld,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%lf,%d", &k, &d[0],
+                 &d[1], &d[2], &c[0], &c[1], &c[2], &c[3], &form, &v[0], &v[1], &v[2],
+                 &v[3], &v[4], &ms, &valid) != 16)
+        continue;  // header or foreign line
+      const int64_t dd[3] = {d[0], d[1], d[2]};
+      const int32_t cc[4] = {c[0], c[1], c[2], c[3]};
+      g_db.rows[db_key(k, dd, cc)].push_back(DbRow{{v[0], v[1], v[2], v[3], v[4]}, ms, valid});
+    }
+    fclose(f);
+  }
+  auto it = g_db.rows.find(db_key(kind, dims, ctx));
+  if (it != g_db.rows.end()) out = it->second;
+  return out;
+}
+
+// The recorded row of one variant (the latest, if recorded twice).
+const DbRow *db_find(const std::vector<DbRow> &rows, const int32_t *var) {
+  const DbRow *hit = nullptr;
+  for (const DbRow &r : rows)
+    if (!memcmp(r.var, var, sizeof r.var)) hit = &r;
+  return hit;
+}
+
 // FP64 check of rows [m0, m0+mw) of an SGEMM output.
 hmlStatus check_window(const hmlGemmShape &s, int64_t lda, int64_t ldb,
                        int64_t ldc, const float *A, const float *B,
@@ -246,10 +327,126 @@ double accept_threshold(int64_t K) {
 
 }  // namespace
 
+// ---- summary tables ------------------------------------------------------
+// The reference prints, after its online sweep, per kernel type a table of
+// the best GFLOPS over row tiles for every (K, N) and a table of max / min
+// GFLOPS over them (hmlSgemmKernelConfigPrintStats,
+// src/hml_sgemm_kernel.cu:318-354; the SGEMV sweep prints the same per N,
+// src/hml_sgemv_kernel.cu:391-632).  Here the tables come from a timing
+// database: per layout group (SGEMM: M, order, transA, transB, beta == 0;
+// SGEMV: order, trans) and per family, rows K (SGEMV: M) and columns N, the
+// best GFLOP/s over the family's variants and the spread max / min of its
+// valid variants in percent; a last table gives the overall winner's
+// family and GFLOP/s.
+const char *family_name(int kind, int f) {
+  static const char *gemm[] = {"basic", "dot", "axpy", "gemv", "bf16x3",
+                               "dense", "bf16x6", "dotb", "gemvn"};
+  static const char *gemv[] = {"basic", "dot", "axpy"};
+  if (kind == 0) return f >= 0 && f < 9 ? gemm[f] : "?";
+  return f >= 0 && f < 3 ? gemv[f] : "?";
+}
+
+void print_stats(const char *db_path, int kind, FILE *out) {
+  const int64_t none[3] = {0, 0, 0};
+  const int32_t nctx[4] = {0, 0, 0, 0};
+  (void)db_prior(db_path, kind, none, nctx);  // refresh the index
+  struct Cell {
+    double best = 0.0, lo = 0.0, hi = 0.0;
+  };
+  // group -> family -> (row, col) -> cell; group -> (row, col) -> (best, family)
+  std::map<std::string, std::map<int, std::map<std::pair<int64_t, int64_t>, Cell>>> tab;
+  std::map<std::string, std::map<std::pair<int64_t, int64_t>, std::pair<double, int>>> win;
+  std::lock_guard<std::mutex> lk(g_db_mutex);
+  for (const auto &kv : g_db.rows) {
+    int k, c[4];
+    long long d[3];
+    if (sscanf(kv.first.c_str(), "%d,%lld,%lld,%lld,%d,%d,%d,%d", &k, &d[0], &d[1], &d[2],
+               &c[0], &c[1], &c[2], &c[3]) != 8 || k != kind)
+      continue;
+    char g[96];
+    int64_t row, col;
+    double flops;
+    if (kind == 0) {
+      snprintf(g, sizeof g, "M=%lld order=%d transA=%d transB=%d beta0=%d", d[0], c[0], c[1],
+               c[2], c[3]);
+      row = d[2]; col = d[1];
+      flops = 2.0 * d[0] * d[1] * d[2];
+    } else {
+      snprintf(g, sizeof g, "order=%d trans=%d", c[0], c[1]);
+      row = d[0]; col = d[1];
+      flops = 2.0 * d[0] * d[1] + 2.0 * d[0];
+    }
+    for (const DbRow &r : kv.second) {
+      if (!r.valid || r.ms <= 0) continue;
+      const double gf = flops / (r.ms * 1e6);
+      Cell &cell = tab[g][r.var[0]][{row, col}];
+      if (cell.best == 0.0) cell.lo = gf;
+      cell.best = std::max(cell.best, gf);
+      cell.hi = std::max(cell.hi, gf);
+      cell.lo = std::min(cell.lo, gf);
+      auto &w = win[g][{row, col}];
+      if (gf > w.first) w = {gf, r.var[0]};
+    }
+  }
+  const char *rowname = kind == 0 ? "K" : "M";
+  for (const auto &gt : tab) {
+    std::set<int64_t> rows, cols;
+    for (const auto &ft : gt.second)
+      for (const auto &c : ft.second) { rows.insert(c.first.first); cols.insert(c.first.second); }
+    auto header = [&](const char *title) {
+      fprintf(out, "%s [%s]:\n%8s", title, gt.first.c_str(), rowname);
+      for (int64_t n : cols) fprintf(out, " %9s", ("N=" + std::to_string(n)).c_str());
+      fprintf(out, "\n");
+      for (size_t i = 0; i < 8 + 10 * cols.size(); ++i) fputc('=', out);
+      fprintf(out, "\n");
+    };
+    for (const auto &ft : gt.second) {
+      char title[64];
+      snprintf(title, sizeof title, "%s GFLOPS", family_name(kind, ft.first));
+      header(title);
+      for (int64_t r : rows) {
+        fprintf(out, "%8lld", (long long)r);
+        for (int64_t n : cols) {
+          auto it = ft.second.find({r, n});
+          if (it == ft.second.end()) fprintf(out, " %9s", "-");
+          else fprintf(out, " %9.2f", it->second.best);
+        }
+        fprintf(out, "\n");
+      }
+      snprintf(title, sizeof title, "%s max / min", family_name(kind, ft.first));
+      header(title);
+      for (int64_t r : rows) {
+        fprintf(out, "%8lld", (long long)r);
+        for (int64_t n : cols) {
+          auto it = ft.second.find({r, n});
+          if (it == ft.second.end() || it->second.lo <= 0) fprintf(out, " %9s", "-");
+          else fprintf(out, " %8.0f%%", it->second.hi * 100.0 / it->second.lo);
+        }
+        fprintf(out, "\n");
+      }
+    }
+    header("best GFLOPS (family)");
+    for (int64_t r : rows) {
+      fprintf(out, "%8lld", (long long)r);
+      for (int64_t n : cols) {
+        auto it = win[gt.first].find({r, n});
+        if (it == win[gt.first].end()) {
+          fprintf(out, " %9s", "-");
+        } else {
+          char cell[32];
+          snprintf(cell, sizeof cell, "%.1f(%d)", it->second.first, it->second.second);
+          fprintf(out, " %9s", cell);
+        }
+      }
+      fprintf(out, "\n");
+    }
+  }
+}
+
 hmlStatus autotune_sgemm_shape(const hmlGemmShape &s, int trials, int flush,
                                uint64_t seed, hmlSgemmVariant *best,
                                double *best_ms, double *heur_ms, FILE *db,
-                               int verbose) {
+                               const char *db_path, int verbose) {
   if (s.M <= 0 || s.N <= 0 || s.K <= 0)
     return fail(HML_STATUS_INVALID_VALUE, "autotune shape must be positive");
   if (trials < 1) trials = 1;
@@ -261,21 +458,45 @@ hmlStatus autotune_sgemm_shape(const hmlGemmShape &s, int trials, int flush,
   const int32_t ctx[4] = {s.order, s.transA, s.transB, s.beta0};
   double bestt = -1.0, heurt = -1.0;
   hmlSgemmVariant bestv = h;
-
-  if (g_fake_timer) {
-    for (const hmlSgemmVariant &v : cands) {
-      const int32_t var[5] = {v.family, v.rt, v.ct, v.ksplit, v.wpb};
-      const double t = g_fake_timer(0, dims, var, g_fake_user);
-      db_write(db, 0, dims, ctx, key.form, var, t, t >= 0 ? 1 : 0);
-      if (t < 0) continue;
-      if (v.family == h.family && v.rt == h.rt && v.ct == h.ct && v.ksplit == h.ksplit && v.wpb == h.wpb) heurt = t;
-      if (bestt < 0 || t < bestt) { bestt = t; bestv = v; }
+  auto same = [](const hmlSgemmVariant &a, const hmlSgemmVariant &b) {
+    return a.family == b.family && a.rt == b.rt && a.ct == b.ct && a.ksplit == b.ksplit &&
+           a.wpb == b.wpb;
+  };
+  // Resume: candidates recorded by an earlier run count with their recorded
+  // median (or failure) and are not measured again.
+  const std::vector<DbRow> prior = db_prior(db_path, 0, dims, ctx);
+  std::vector<hmlSgemmVariant> todo;
+  for (const hmlSgemmVariant &v : cands) {
+    const int32_t var[5] = {v.family, v.rt, v.ct, v.ksplit, v.wpb};
+    const DbRow *r = db_find(prior, var);
+    if (!r) {
+      todo.push_back(v);
+      continue;
     }
-    if (bestt < 0) return fail(HML_STATUS_VALIDATION_FAILED, "no candidate succeeded");
+    const double t = r->valid ? r->ms : -1.0;
+    if (t < 0) continue;
+    if (same(v, h)) heurt = t;
+    if (bestt < 0 || t < bestt) { bestt = t; bestv = v; }
+  }
+  auto finish = [&]() -> hmlStatus {
+    if (bestt < 0) return fail(HML_STATUS_VALIDATION_FAILED, "no candidate passed validation");
     *best = bestv;
     if (best_ms) *best_ms = bestt;
     if (heur_ms) *heur_ms = heurt;
     return HML_STATUS_SUCCESS;
+  };
+  if (todo.empty()) return finish();
+
+  if (g_fake_timer) {
+    for (const hmlSgemmVariant &v : todo) {
+      const int32_t var[5] = {v.family, v.rt, v.ct, v.ksplit, v.wpb};
+      const double t = g_fake_timer(0, dims, var, g_fake_user);
+      db_write(db, 0, dims, ctx, key.form, var, t, t >= 0 ? 1 : 0);
+      if (t < 0) continue;
+      if (same(v, h)) heurt = t;
+      if (bestt < 0 || t < bestt) { bestt = t; bestv = v; }
+    }
+    return finish();
   }
 
   int64_t lda, ldb, ldc;
@@ -330,7 +551,7 @@ hmlStatus autotune_sgemm_shape(const hmlGemmShape &s, int trials, int flush,
     }
   }
   const double thr = accept_threshold(s.K);
-  for (const hmlSgemmVariant &v : cands) {
+  for (const hmlSgemmVariant &v : todo) {
     const int32_t var[5] = {v.family, v.rt, v.ct, v.ksplit, v.wpb};
     e = hipMemcpy(C.p, C0.p, nC * 4, hipMemcpyDeviceToDevice);
     if (e != hipSuccess) return from_hip(e, "autotune copy");
@@ -365,25 +586,23 @@ hmlStatus autotune_sgemm_shape(const hmlGemmShape &s, int trials, int flush,
               v.family, v.rt, v.ct, v.ksplit, v.wpb, (long long)s.M, (long long)s.N, (long long)s.K);
     }
     db_write(db, 0, dims, ctx, key.form, var, t, t >= 0 ? 1 : 0);
+    if (db) fflush(db);
     if (t < 0) continue;
-    if (v.family == h.family && v.rt == h.rt && v.ct == h.ct && v.ksplit == h.ksplit && v.wpb == h.wpb) heurt = t;
+    if (same(v, h)) heurt = t;
     if (bestt < 0 || t < bestt) { bestt = t; bestv = v; }
     if (verbose >= 2)
       fprintf(stderr, "Info: kernel[%lld][%lld][%lld][%d %d %d %d %d] = %.4f ms\n",
               (long long)s.M, (long long)s.N, (long long)s.K, v.family, v.rt,
               v.ct, v.ksplit, v.wpb, t);
   }
-  if (bestt < 0) return fail(HML_STATUS_VALIDATION_FAILED, "no candidate passed validation");
-  *best = bestv;
-  if (best_ms) *best_ms = bestt;
-  if (heur_ms) *heur_ms = heurt;
-  return HML_STATUS_SUCCESS;
+  return finish();
 }
 
 hmlStatus autotune_sgemv_shape(hmlOrder order, hmlOp trans, int64_t M,
                                int64_t N, int trials, int flush, uint64_t seed,
                                hmlSgemvVariant *best, double *best_ms,
-                               double *heur_ms, FILE *db, int verbose) {
+                               double *heur_ms, FILE *db, const char *db_path,
+                               int verbose) {
   if (M <= 0 || N <= 0) return fail(HML_STATUS_INVALID_VALUE, "autotune shape must be positive");
   if (trials < 1) trials = 1;
   const GemvKey key = gemv_key(order, trans, M, N, 1);
@@ -397,8 +616,30 @@ hmlStatus autotune_sgemv_shape(hmlOrder order, hmlOp trans, int64_t M,
   auto same = [](const hmlSgemvVariant &a, const hmlSgemvVariant &b) {
     return a.family == b.family && a.lpr == b.lpr && a.rb == b.rb && a.split == b.split && a.wpb == b.wpb;
   };
+  const std::vector<DbRow> prior = db_prior(db_path, 1, dims, ctx);
+  std::vector<hmlSgemvVariant> todo;
+  for (const hmlSgemvVariant &v : cands) {
+    const int32_t var[5] = {v.family, v.lpr, v.rb, v.split, v.wpb};
+    const DbRow *r = db_find(prior, var);
+    if (!r) {
+      todo.push_back(v);
+      continue;
+    }
+    const double t = r->valid ? r->ms : -1.0;
+    if (t < 0) continue;
+    if (same(v, h)) heurt = t;
+    if (bestt < 0 || t < bestt) { bestt = t; bestv = v; }
+  }
+  auto finish = [&]() -> hmlStatus {
+    if (bestt < 0) return fail(HML_STATUS_VALIDATION_FAILED, "no candidate passed validation");
+    *best = bestv;
+    if (best_ms) *best_ms = bestt;
+    if (heur_ms) *heur_ms = heurt;
+    return HML_STATUS_SUCCESS;
+  };
+  if (todo.empty()) return finish();
   if (g_fake_timer) {
-    for (const hmlSgemvVariant &v : cands) {
+    for (const hmlSgemvVariant &v : todo) {
       const int32_t var[5] = {v.family, v.lpr, v.rb, v.split, v.wpb};
       const double t = g_fake_timer(1, dims, var, g_fake_user);
       db_write(db, 1, dims, ctx, key.family, var, t, t >= 0 ? 1 : 0);
@@ -406,11 +647,7 @@ hmlStatus autotune_sgemv_shape(hmlOrder order, hmlOp trans, int64_t M,
       if (same(v, h)) heurt = t;
       if (bestt < 0 || t < bestt) { bestt = t; bestv = v; }
     }
-    if (bestt < 0) return fail(HML_STATUS_VALIDATION_FAILED, "no candidate succeeded");
-    *best = bestv;
-    if (best_ms) *best_ms = bestt;
-    if (heur_ms) *heur_ms = heurt;
-    return HML_STATUS_SUCCESS;
+    return finish();
   }
   const int64_t lda = order == HML_ROW_MAJOR ? N : M;
   const size_t nA = (size_t)M * N;
@@ -450,7 +687,7 @@ hmlStatus autotune_sgemv_shape(hmlOrder order, hmlOp trans, int64_t M,
       return fail(HML_STATUS_VALIDATION_FAILED, "reference GEMV failed the FP64 check");
   }
   const double thr = accept_threshold(nx);
-  for (const hmlSgemvVariant &v : cands) {
+  for (const hmlSgemvVariant &v : todo) {
     const int32_t var[5] = {v.family, v.lpr, v.rb, v.split, v.wpb};
     st = run(v, y.f());
     if (st == HML_STATUS_SUCCESS && (e = hipDeviceSynchronize()) != hipSuccess)
@@ -480,21 +717,27 @@ hmlStatus autotune_sgemv_shape(hmlOrder order, hmlOp trans, int64_t M,
               v.family, v.lpr, v.rb, v.split, v.wpb);
     }
     db_write(db, 1, dims, ctx, key.family, var, t, t >= 0 ? 1 : 0);
+    if (db) fflush(db);
     if (t < 0) continue;
     if (same(v, h)) heurt = t;
     if (bestt < 0 || t < bestt) { bestt = t; bestv = v; }
   }
   (void)y64;
-  if (bestt < 0) return fail(HML_STATUS_VALIDATION_FAILED, "no candidate passed validation");
-  *best = bestv;
-  if (best_ms) *best_ms = bestt;
-  if (heur_ms) *heur_ms = heurt;
-  return HML_STATUS_SUCCESS;
+  return finish();
+}
+
+// Database of the online sweeps: $HML_AUTOTUNE_DB when set (the sweep then
+// resumes from it and appends to it), else none.
+const char *online_db_path() {
+  const char *p = getenv("HML_AUTOTUNE_DB");
+  return p && *p ? p : nullptr;
 }
 
 hmlStatus autotune_sgemm_online(int verbose) {
   // Row-major NN grid (the layout of the reference API): M 2^12..2^20,
   // N 1..128, K 2^5..2^13, at most 1 GiB per operand.
+  const char *db_path = online_db_path();
+  FILE *db = db_open(db_path);
   std::vector<SgemmRuleV2> rules;
   for (int ml = 12; ml <= 20; ml += 2)
     for (int nl = 0; nl <= 7; ++nl)
@@ -504,8 +747,11 @@ hmlStatus autotune_sgemm_online(int verbose) {
         if ((double)s.M * s.K * 4 > (double)(1 << 30)) continue;
         hmlSgemmVariant best;
         double bt = 0, ht = 0;
-        hmlStatus st = autotune_sgemm_shape(s, 5, 1, 1, &best, &bt, &ht, nullptr, verbose);
-        if (st != HML_STATUS_SUCCESS) return st;
+        hmlStatus st = autotune_sgemm_shape(s, 5, 1, 1, &best, &bt, &ht, db, db_path, verbose);
+        if (st != HML_STATUS_SUCCESS) {
+          if (db) fclose(db);
+          return st;
+        }
         const GemmKey k = shape_key(s);
         rules.push_back(SgemmRuleV2{k.form, ilog2_floor(k.M), ilog2_floor(k.N), ilog2_floor(k.K), best});
         if (verbose)
@@ -513,11 +759,15 @@ hmlStatus autotune_sgemm_online(int verbose) {
                   (long long)s.M, (long long)s.N, (long long)s.K, best.family,
                   best.rt, best.ct, best.ksplit, best.wpb, bt);
       }
+  if (db) fclose(db);
   config_set_sgemm_v2(rules);
+  if (verbose && db_path) print_stats(db_path, 0, stderr);
   return HML_STATUS_SUCCESS;
 }
 
 hmlStatus autotune_sgemv_online(int verbose) {
+  const char *db_path = online_db_path();
+  FILE *db = db_open(db_path);
   std::vector<SgemvRuleV2> rules;
   for (int trans = 0; trans <= 1; ++trans)
     for (int ml = 10; ml <= 20; ml += 2)
@@ -527,8 +777,11 @@ hmlStatus autotune_sgemv_online(int verbose) {
         hmlSgemvVariant best;
         double bt = 0, ht = 0;
         hmlStatus st = autotune_sgemv_shape(HML_ROW_MAJOR, (hmlOp)trans, M, N, 5, 1, 1,
-                                            &best, &bt, &ht, nullptr, verbose);
-        if (st != HML_STATUS_SUCCESS) return st;
+                                            &best, &bt, &ht, db, db_path, verbose);
+        if (st != HML_STATUS_SUCCESS) {
+          if (db) fclose(db);
+          return st;
+        }
         const GemvKey k = gemv_key(HML_ROW_MAJOR, (hmlOp)trans, M, N, 1);
         rules.push_back(SgemvRuleV2{k.family, ilog2_floor(k.outputs), ilog2_floor(k.reduced), best});
         if (verbose)
@@ -536,7 +789,9 @@ hmlStatus autotune_sgemv_online(int verbose) {
                   (long long)M, (long long)N, trans, best.family, best.lpr,
                   best.rb, best.split, best.wpb, bt);
       }
+  if (db) fclose(db);
   config_set_sgemv_v2(rules);
+  if (verbose && db_path) print_stats(db_path, 1, stderr);
   return HML_STATUS_SUCCESS;
 }
 
@@ -576,7 +831,7 @@ hmlStatus hmlSgemmAutotune(const hmlGemmShape *shape, int32_t trials,
   FILE *db = db_open(db_path);
   if (db_path && !db) return fail(HML_STATUS_IO_ERROR, "cannot open '%s'", db_path);
   hmlStatus s = autotune_sgemm_shape(*shape, trials, flush, seed, best, best_ms,
-                                     heuristic_ms, db, verbosity() >= 2 ? 2 : 0);
+                                     heuristic_ms, db, db_path, verbosity() >= 2 ? 2 : 0);
   if (db) fclose(db);
   return s;
 }
@@ -590,9 +845,24 @@ hmlStatus hmlSgemvAutotune(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
   FILE *db = db_open(db_path);
   if (db_path && !db) return fail(HML_STATUS_IO_ERROR, "cannot open '%s'", db_path);
   hmlStatus s = autotune_sgemv_shape(order, trans, M, N, trials, flush, seed, best,
-                                     best_ms, heuristic_ms, db, verbosity() >= 2 ? 2 : 0);
+                                     best_ms, heuristic_ms, db, db_path,
+                                     verbosity() >= 2 ? 2 : 0);
   if (db) fclose(db);
   return s;
+}
+
+hmlStatus hmlAutotuneStats(const char *db_path, int32_t kind, const char *out_path) {
+  clear_error();
+  if (!db_path || (kind != 0 && kind != 1))
+    return fail(HML_STATUS_INVALID_VALUE, "autotune stats: db_path and kind 0 or 1 required");
+  FILE *probe = fopen(db_path, "r");
+  if (!probe) return fail(HML_STATUS_IO_ERROR, "cannot open '%s'", db_path);
+  fclose(probe);
+  FILE *out = out_path ? fopen(out_path, "w") : stderr;
+  if (!out) return fail(HML_STATUS_IO_ERROR, "cannot create '%s'", out_path);
+  print_stats(db_path, kind, out);
+  if (out != stderr) fclose(out);
+  return HML_STATUS_SUCCESS;
 }
 
 hmlStatus hmlSgemmConfigAddRule(const hmlGemmShape *shape,

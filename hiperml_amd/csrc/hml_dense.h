@@ -24,12 +24,24 @@ struct DenseParams {
   int64_t M, N, Mp, Np, Kp;
   float alpha, beta;
   int32_t read_c;
+  // scaled fp16 split only: per-row maxima (FP32 bits) of A and of B^T,
+  // from which both the split and the epilogue derive the scale exponents
+  const uint32_t *rmaxA;
+  const uint32_t *rmaxB;
 };
 
 // S: 0 = FP32 copy, 2 = hi/lo bf16, 3 = hi/mid/lo bf16.
 hipError_t launch_split(const float *src, int64_t sR, int64_t sK, int64_t rows,
                         int64_t K, int64_t rowsP, int64_t Kp, int S, void *dst,
                         hipStream_t st);
+// S: 0 FP32, 2 bf16x3, 3 bf16x6, 4 scaled f16x3.
 hipError_t launch_dense(int S, const DenseParams &p, hipStream_t st);
+// rowmax[r] = max_k |X(r,k)| as FP32 bits; rowmax must be zero on entry.
+hipError_t launch_rowmax(const float *src, int64_t sR, int64_t sK, int64_t rows,
+                         int64_t K, uint32_t *rowmax, hipStream_t st);
+// Scaled fp16 hi / lo planes (k-blocked, 2 planes) from the row maxima.
+hipError_t launch_split_f16(const float *src, int64_t sR, int64_t sK, int64_t rows,
+                            int64_t K, int64_t rowsP, int64_t Kp, const uint32_t *rowmax,
+                            void *dst, hipStream_t st);
 
 }  // namespace hml

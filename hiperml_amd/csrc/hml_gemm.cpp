@@ -338,13 +338,24 @@ hmlStatus run_gemv_family(const GemmCanon &g, const GemmKey &k,
 
 // Strides too large for the 31-bit buffer offsets of the skinny and
 // multi-vector kernels: plain FP32 dense tiles instead (they address with
-// 64-bit pointers through the split pass).
+// 64-bit pointers through the split pass).  A caller's workspace is used
+// when it holds the dense kernel's operand copies; without one, the
+// library workspace of the stream.
 hmlStatus run_dense_fallback(const GemmCanon &g, const GemmKey &k, float alpha, float beta,
-                             int read_c, hipStream_t st) {
+                             int read_c, void *caller_ws, size_t ws_bytes, hipStream_t st) {
   const size_t dn = dense_workspace(0, k.M, k.N, k.K);
-  void *dws = nullptr;
-  hmlStatus s = library_workspace(dn, &dws);
-  if (s != HML_STATUS_SUCCESS) return s;
+  WsLease lease;
+  void *dws = caller_ws;
+  if (caller_ws) {
+    if (ws_bytes < dn)
+      return fail(HML_STATUS_WORKSPACE_TOO_SMALL,
+                  "strides exceed the skinny kernels' 31-bit offsets; the dense fallback "
+                  "needs a workspace of %zu bytes (got %zu)", dn, ws_bytes);
+  } else {
+    hmlStatus s = library_workspace(dn, st, &lease);
+    if (s != HML_STATUS_SUCCESS) return s;
+    dws = lease.ptr();
+  }
   return run_dense(0, g.A, g.sAm, g.sAk, g.B, g.sBk, g.sBn, g.C, g.sCm, g.sCn, g.M, g.N, g.K,
                    alpha, beta, read_c, dws, dn, st);
 }
@@ -353,9 +364,13 @@ hmlStatus run_sgemm(const GemmCanon &g, const GemmKey &k,
                     const hmlSgemmVariant &v, float alpha, float beta,
                     int read_c, void *ws, size_t ws_bytes, hipStream_t st) {
   const size_t need = workspace_bytes(k, v);
+  void *const caller_ws = ws;
+  WsLease lease;  // held until the kernels below are queued
   if (need > 0 && !ws) {
-    hmlStatus s = library_workspace(need, &ws);
+    hmlStatus s = library_workspace(need, st, &lease);
     if (s != HML_STATUS_SUCCESS) return s;
+    ws = lease.ptr();
+    ws_bytes = need;
   } else if (need > ws_bytes) {
     return fail(HML_STATUS_WORKSPACE_TOO_SMALL, "workspace %zu < %zu bytes",
                 ws_bytes, need);
@@ -376,7 +391,7 @@ hmlStatus run_sgemm(const GemmCanon &g, const GemmKey &k,
     p.chunk = sp.chunk; p.split = sp.split; p.wsum = sp.wsum;
     const bool bvec = g.sBk == 1 || g.K == 1;
     if (!gemv_multi_offsets_fit(v.rt, v.ct, (int)g.N, bvec, p))
-      return run_dense_fallback(g, k, alpha, beta, read_c, st);
+      return run_dense_fallback(g, k, alpha, beta, read_c, caller_ws, ws_bytes, st);
     hipError_t e = launch_gemv_multi(v.rt, v.ct, (int)g.N, bvec, v.wpb, p, st);
     if (e != hipSuccess) return from_hip(e, "multi-vector gemv launch");
     if (slots > 1) {
@@ -391,7 +406,8 @@ hmlStatus run_sgemm(const GemmCanon &g, const GemmKey &k,
                      read_c, ws, ws ? need : 0, st);
 
   const SkinnyPlan pl = plan_skinny(k, v);
-  if (!skinny_offsets_fit(g, v, pl)) return run_dense_fallback(g, k, alpha, beta, read_c, st);
+  if (!skinny_offsets_fit(g, v, pl))
+    return run_dense_fallback(g, k, alpha, beta, read_c, caller_ws, ws_bytes, st);
   hipError_t e;
   if (k.form == kFormDot) {
     DotParams p{};
@@ -561,21 +577,25 @@ size_t hmlSgemmWorkspaceSize(hmlOrder order, hmlOp transA, hmlOp transB,
                              const hmlSgemmVariant *variant) {
   if (M <= 0 || N <= 0 || K <= 0) return 0;
   GemmCanon g = dense_canon(order, transA, transB, M, N, K);
-  // The strides are unknown here, so when N == 1 after orientation the call
-  // may take either the GEMV or the skinny route: return the larger need.
-  GemmKey key = make_key(g, 0.0f, 0, order, transA, transB);
+  // The strides and beta are unknown here: when N == 1 after orientation
+  // the call may take either the GEMV or the skinny route, and a selector
+  // may choose differently for beta == 0 and beta != 0.  Return the largest
+  // need over those cases.
   size_t need = 0;
-  for (int gemv = 0; gemv <= (g.N == 1 ? 1 : 0); ++gemv) {
-    key.gemv_ok = gemv == 1;
-    hmlSgemmVariant v;
-    if (variant) {
-      v = *variant;
-    } else {
-      int src;
-      select_sgemm(key, &v, &src);
+  for (int beta_nz = 0; beta_nz <= 1; ++beta_nz) {
+    GemmKey key = make_key(g, beta_nz ? 1.0f : 0.0f, 0, order, transA, transB);
+    for (int gemv = 0; gemv <= (g.N == 1 ? 1 : 0); ++gemv) {
+      key.gemv_ok = gemv == 1;
+      hmlSgemmVariant v;
+      if (variant) {
+        v = *variant;
+      } else {
+        int src;
+        select_sgemm(key, &v, &src);
+      }
+      if (!variant || sgemm_variant_runnable(key, v))
+        need = std::max(need, workspace_bytes(key, v));
     }
-    if (!variant || sgemm_variant_runnable(key, v))
-      need = std::max(need, workspace_bytes(key, v));
   }
   return need;
 }

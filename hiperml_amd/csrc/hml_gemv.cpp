@@ -214,10 +214,12 @@ hmlStatus sgemv_impl(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
   p.alpha = alpha; p.beta = beta; p.read_c = read_c;
   const int slots = gemv_resolve_split(c.family, c.reduced, v.lpr, v.split, v.wpb, &p);
   const size_t need = slots > 1 ? align256((size_t)slots * c.outputs * sizeof(float)) : 0;
+  WsLease lease;  // held until the kernels below are queued
   if (need > 0) {
     if (!ws) {
-      hmlStatus s = library_workspace(need, &ws);
+      hmlStatus s = library_workspace(need, st, &lease);
       if (s != HML_STATUS_SUCCESS) return s;
+      ws = lease.ptr();
     } else if (ws_bytes < need) {
       return fail(HML_STATUS_WORKSPACE_TOO_SMALL, "workspace %zu < %zu bytes", ws_bytes, need);
     }
@@ -301,16 +303,22 @@ size_t hmlSgemvWorkspaceSize(hmlOrder order, hmlOp trans, int64_t M,
                              int64_t N, const hmlSgemvVariant *variant) {
   if (M <= 0 || N <= 0) return 0;
   GemvCanon c = canonicalize(order, trans, M, N, nullptr, 1, nullptr, nullptr);
-  GemvKey key{c.family, c.outputs, c.reduced, 0};
-  hmlSgemvVariant v;
-  if (variant) {
-    v = *variant;
-  } else {
-    int src;
-    select_sgemv(key, &v, &src);
+  // beta is unknown here and a selector may choose differently for
+  // beta == 0: the larger need of the two.  The launch may use fewer splits
+  // than requested, never more.
+  size_t need = 0;
+  for (int beta0 = 0; beta0 <= 1; ++beta0) {
+    GemvKey key{c.family, c.outputs, c.reduced, beta0};
+    hmlSgemvVariant v;
+    if (variant) {
+      v = *variant;
+    } else {
+      int src;
+      select_sgemv(key, &v, &src);
+    }
+    need = std::max(need, gemv_workspace(key, v));
   }
-  // The launch may use fewer splits than requested, never more.
-  return gemv_workspace(key, v);
+  return need;
 }
 
 hmlStatus hmlSgemvSelect(hmlOrder order, hmlOp trans, int64_t M, int64_t N,

@@ -148,24 +148,38 @@ hmlStatus hmlSgemmGroupedRun(hmlGroupedPlan_t *plan, int32_t use_graph,
   if (!use_graph) return record(plan, st);
   if (!plan->exec) {
     // Capture on a private stream (the null stream cannot be captured).
-    hipError_t e = hipStreamCreate(&plan->graph_stream);
-    if (e != hipSuccess) return from_hip(e, "hipStreamCreate");
-    e = hipStreamBeginCapture(plan->graph_stream, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) return from_hip(e, "hipStreamBeginCapture");
-    hmlStatus s = record(plan, plan->graph_stream);
+    // Any failure leaves the plan as before: no stream, graph or exec.
+    hipStream_t cs = nullptr;
     hipGraph_t g = nullptr;
-    e = hipStreamEndCapture(plan->graph_stream, &g);
+    hipGraphExec_t x = nullptr;
+    hipError_t e = hipStreamCreate(&cs);
+    if (e != hipSuccess) return from_hip(e, "hipStreamCreate");
+    hmlStatus s = HML_STATUS_SUCCESS;
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      s = from_hip(e, "hipStreamBeginCapture");
+    } else {
+      s = record(plan, cs);
+      e = hipStreamEndCapture(cs, &g);
+      if (s == HML_STATUS_SUCCESS && e != hipSuccess) s = from_hip(e, "hipStreamEndCapture");
+      if (s == HML_STATUS_SUCCESS && !g)
+        s = fail(HML_STATUS_HIP_ERROR, "hipStreamEndCapture returned no graph");
+    }
+    if (s == HML_STATUS_SUCCESS) {
+      e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        x = nullptr;
+        s = from_hip(e, "hipGraphInstantiate");
+      }
+    }
     if (s != HML_STATUS_SUCCESS) {
       if (g) (void)hipGraphDestroy(g);
+      (void)hipStreamDestroy(cs);
       return s;
     }
-    if (e != hipSuccess) return from_hip(e, "hipStreamEndCapture");
+    plan->graph_stream = cs;
     plan->graph = g;
-    e = hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-      plan->exec = nullptr;
-      return from_hip(e, "hipGraphInstantiate");
-    }
+    plan->exec = x;
   }
   return from_hip(hipGraphLaunch(plan->exec, st), "hipGraphLaunch");
 }

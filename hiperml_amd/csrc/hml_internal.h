@@ -156,6 +156,8 @@ hipError_t launch_check_f64(const CheckParams &p, int64_t *nblocks,
                             hipStream_t st);
 
 hipError_t launch_fill(float *dst, size_t n, float value, hipStream_t st);
+// Reads `bytes` (whole 16-byte chunks) and writes nothing (see k_read_probe).
+hipError_t launch_read_probe(const void *src, size_t bytes, uint32_t *sink, hipStream_t st);
 
 }  // namespace hml
 

@@ -153,7 +153,39 @@ __global__ __launch_bounds__(256) void k_compare(const float *C,
   if (t < 5) stats[(int64_t)blockIdx.x * 5 + t] = red[t][0];
 }
 
+// Read-only stream: every thread reads 16-byte chunks (4 independent loads
+// per step, grid-stride) and folds them with XOR; the fold is stored only in
+// the never-taken case that it equals a fixed odd constant, so the kernel
+// moves no bytes except the reads.
+typedef uint32_t probe_u4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_read_probe(const probe_u4 *__restrict__ src,
+                                                    size_t n16, uint32_t *sink) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  probe_u4 acc = {0u, 0u, 0u, 0u};
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    const probe_u4 a = __builtin_nontemporal_load(src + i);
+    const probe_u4 b = __builtin_nontemporal_load(src + i + stride);
+    const probe_u4 c = __builtin_nontemporal_load(src + i + 2 * stride);
+    const probe_u4 d = __builtin_nontemporal_load(src + i + 3 * stride);
+    acc ^= a ^ b ^ c ^ d;
+  }
+  for (; i < n16; i += stride) acc ^= __builtin_nontemporal_load(src + i);
+  const uint32_t f = acc.x ^ acc.y ^ acc.z ^ acc.w;
+  if (f == 0x9E3779B9u) sink[0] = f;
+}
+
 }  // namespace
+
+hipError_t launch_read_probe(const void *src, size_t bytes, uint32_t *sink, hipStream_t st) {
+  const size_t n16 = bytes / 16;
+  if (n16 == 0) return hipSuccess;
+  // 8 workgroups of 4 waves per CU on 256 CUs
+  hipLaunchKernelGGL(k_read_probe, dim3(2048), dim3(256), 0, st, (const probe_u4 *)src, n16,
+                     sink);
+  return hipGetLastError();
+}
 
 hipError_t launch_fill_uniform(float *dst, size_t n, uint64_t seed, float lo,
                                float hi, hipStream_t st) {

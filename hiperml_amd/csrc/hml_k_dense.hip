@@ -13,6 +13,19 @@
 // are exact in FP32, so the error comes from the dropped terms
 // (2^-16 relative for S = 2, ~2^-24 for S = 3) plus FP32 accumulation.
 //
+// Scaled fp16 split ("f16x3"; Ootomo & Yokota, "Recovering single precision
+// accuracy from Tensor Cores while surpassing the FP32 theoretical peak
+// performance", IJHPCA 2022): the same three products h*h + h*l + l*h, but
+// with fp16 terms, whose 11-bit significands make the two-term split 22 bits
+// wide (bf16: 16), so the representation error per operand is ~2^-22
+// instead of ~2^-16.  fp16's narrow exponent range is handled by exact
+// power-of-two scaling: row r of A (column n of B) is multiplied by 2^e_r
+// with e_r chosen so that its largest magnitude lands in [2^14, 2^15), and
+// the epilogue multiplies by 2^-(e_m + e_n).  The correction products
+// h*l + l*h accumulate separately from h*h (their roundings then act on a
+// 2^-11 smaller sum) and the two sums meet once, in the epilogue.
+// v_mfma_f32_16x16x32_f16 runs at the bf16 rate.
+//
 // Data path: a split pass writes zero-padded bf16 planes in a k-blocked
 // layout, Ap[S][Kp/32][Mp][32] and Bp[S][Kp/32][Np][32] (B transposed): the
 // 32-wide k-step of a 256-row panel is then 16 KB of contiguous memory, read
@@ -32,6 +45,7 @@ namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 
 constexpr int BM = kDenseTileM, BN = kDenseTileN;   // FP32 kernel tile
@@ -106,6 +120,86 @@ __global__ void k_split(const float *__restrict__ src, int64_t sR, int64_t sK,
   }
 }
 
+// ---- row maxima for the scaled fp16 split ---------------------------------
+// rowmax[r] = max_k |X(r,k)| as FP32 bits (non-negative floats order like
+// their bit patterns, so an integer atomicMax is a float max; NaN ranks
+// above Inf).  rowmax must be zero on entry.  K contiguous (sK == 1): each
+// thread takes 8 consecutive k of one row, the lanes of a wave that share a
+// row combine by a segmented max scan, and the last lane of each row
+// segment issues the atomic.  Otherwise lanes take consecutive rows (sR ==
+// 1 then coalesces) and each thread scans 256 k.
+__global__ void k_rowmax(const float *__restrict__ src, int64_t sR, int64_t sK,
+                         int64_t rows, int64_t K, uint32_t *__restrict__ rowmax,
+                         int kfast) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  uint32_t m = 0;
+  int64_t r;
+  if (kfast) {
+    const int64_t kc_n = (K + 7) / 8;
+    r = idx / kc_n;
+    const int64_t kc = idx - r * kc_n;
+    if (r < rows) {
+      for (int j = 0; j < 8; ++j) {
+        const int64_t k = kc * 8 + j;
+        if (k < K) m = max(m, __float_as_uint(src[r * sR + k]) & 0x7fffffffu);
+      }
+    }
+    // segmented inclusive max scan over lanes of the same row
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t om = __shfl_up(m, off, 64);
+      const int64_t orr = __shfl_up(r, off, 64);
+      if (lane >= off && orr == r) m = max(m, om);
+    }
+    const int64_t nr = __shfl_down(r, 1, 64);
+    const bool last = lane == 63 || nr != r;
+    if (r < rows && last && m) atomicMax(rowmax + r, m);
+    return;
+  }
+  const int64_t nkc = (K + 255) / 256;
+  const int64_t kc = idx / rows;
+  r = idx - kc * rows;
+  if (kc >= nkc) return;
+  const int64_t k1 = min(K, (kc + 1) * 256);
+  for (int64_t k = kc * 256; k < k1; ++k)
+    m = max(m, __float_as_uint(src[r * sR + k * sK]) & 0x7fffffffu);
+  if (m) atomicMax(rowmax + r, m);
+}
+
+// Exponent e with max * 2^e in [2^14, 2^15) (0 for a zero, Inf or NaN max).
+__device__ __forceinline__ int split_exponent(uint32_t maxbits) {
+  const float mx = __uint_as_float(maxbits);
+  if (maxbits == 0 || maxbits >= 0x7f800000u) return 0;
+  return 14 - ilogbf(mx);
+}
+
+// Scaled fp16 hi / lo planes in the k-blocked layout of k_split.
+__global__ void k_split_f16(const float *__restrict__ src, int64_t sR, int64_t sK,
+                            int64_t rows, int64_t K, int64_t rowsP, int64_t Kp,
+                            const uint32_t *__restrict__ rowmax,
+                            void *__restrict__ dst) {
+  const int64_t kc_n = Kp / 8;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rowsP * kc_n) return;
+  const int64_t rq = idx >> 2;
+  const int64_t kt = rq / rowsP;
+  const int64_t r = rq - kt * rowsP;
+  const int64_t kc = kt * 4 + (idx & 3);
+  const int e = r < rows ? split_exponent(rowmax[r]) : 0;
+  _Float16 hi[8], lo[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int64_t k = kc * 8 + j;
+    const float x = (r < rows && k < K) ? ldexpf(src[r * sR + k * sK], e) : 0.f;
+    hi[j] = (_Float16)x;
+    lo[j] = (_Float16)(x - (float)hi[j]);  // the residual is exact in FP32
+  }
+  const int64_t plane = rowsP * Kp;
+  _Float16 *d = (_Float16 *)dst + idx * 8;
+  *reinterpret_cast<h8 *>(d) = h8{hi[0], hi[1], hi[2], hi[3], hi[4], hi[5], hi[6], hi[7]};
+  *reinterpret_cast<h8 *>(d + plane) = h8{lo[0], lo[1], lo[2], lo[3], lo[4], lo[5], lo[6], lo[7]};
+}
+
 // Block index -> (bm, bn): consecutive launch indices rotate over the 8
 // XCDs, so first regroup them per XCD, then walk each XCD's share in
 // column groups of 8 block-rows so its blocks reuse A and B panels in L2.
@@ -147,7 +241,9 @@ __device__ __forceinline__ int sw_off(int x, int kq) {
   return x * kSplitKBlock + 8 * (kq ^ (((x >> 1) ^ (x >> 2)) & 3));
 }
 
-template <int S>
+// S = 2 / 3: bf16 planes, one accumulator; S = 2 with F16: scaled fp16
+// planes, h*h and the corrections in separate accumulators.
+template <int S, bool F16>
 __global__ __launch_bounds__(512) void k_dense_bf16(DenseParams p) {
   __shared__ __attribute__((aligned(16))) uint16_t As[2][S][BMH * kSplitKBlock];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[2][S][BNH * kSplitKBlock];
@@ -187,11 +283,17 @@ __global__ __launch_bounds__(512) void k_dense_bf16(DenseParams p) {
     }
   };
 
-  f4 acc[4][4];
+  f4 acc[4][4], acc2[F16 ? 4 : 1][F16 ? 4 : 1];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  if (F16) {
+#pragma unroll
+    for (int i = 0; i < (F16 ? 4 : 1); ++i)
+#pragma unroll
+      for (int j = 0; j < (F16 ? 4 : 1); ++j) acc2[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  }
 
   // Step t: read the fragments of step t from LDS buffer t % 2, store step
   // t + 1 (in registers since step t - 1) into buffer (t + 1) % 2 -- read
@@ -237,20 +339,49 @@ __global__ __launch_bounds__(512) void k_dense_bf16(DenseParams p) {
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ia][i], fb[ib][j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 4; ++j) {
+          if constexpr (F16) {
+            // q = 0, 1: the corrections l*h, h*l; q = 2: h*h
+            const h8 a = __builtin_bit_cast(h8, fa[ia][i]);
+            const h8 b = __builtin_bit_cast(h8, fb[ib][j]);
+            if (q < 2) acc2[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc2[i][j], 0, 0, 0);
+            else acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc[i][j], 0, 0, 0);
+          } else {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ia][i], fb[ib][j], acc[i][j], 0, 0, 0);
+          }
+        }
     }
     __syncthreads();
   }
 
+  if constexpr (F16) {
+    // undo the row / column scaling: exact powers of two
+    int ea[4][4], eb[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) eb[j] = split_exponent(p.rmaxB[bn * BNH + wc * 64 + j * 16 + r16]);
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-        store_c(p, bm * BMH + wr * 64 + i * 16 + 4 * g + e,
-                bn * BNH + wc * 64 + j * 16 + r16, acc[i][j][e]);
+      for (int e = 0; e < 4; ++e) ea[i][e] = split_exponent(p.rmaxA[bm * BMH + wr * 64 + i * 16 + 4 * g + e]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          store_c(p, bm * BMH + wr * 64 + i * 16 + 4 * g + e,
+                  bn * BNH + wc * 64 + j * 16 + r16,
+                  ldexpf(acc[i][j][e] + acc2[i][j][e], -(ea[i][e] + eb[j])));
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          store_c(p, bm * BMH + wr * 64 + i * 16 + 4 * g + e,
+                  bn * BNH + wc * 64 + j * 16 + r16, acc[i][j][e]);
+  }
 }
 
 // ---- plain FP32 GEMM on v_mfma_f32_16x16x4f32 ------------------------------
@@ -337,15 +468,38 @@ hipError_t launch_split(const float *src, int64_t sR, int64_t sK, int64_t rows,
   return hipGetLastError();
 }
 
+hipError_t launch_rowmax(const float *src, int64_t sR, int64_t sK, int64_t rows,
+                         int64_t K, uint32_t *rowmax, hipStream_t st) {
+  if (rows <= 0 || K <= 0) return hipSuccess;
+  const int kfast = sK == 1 ? 1 : 0;
+  const int64_t total = kfast ? rows * ((K + 7) / 8) : rows * ((K + 255) / 256);
+  const dim3 grid((unsigned)((total + 255) / 256));
+  hipLaunchKernelGGL(k_rowmax, grid, dim3(256), 0, st, src, sR, sK, rows, K, rowmax, kfast);
+  return hipGetLastError();
+}
+
+hipError_t launch_split_f16(const float *src, int64_t sR, int64_t sK, int64_t rows,
+                            int64_t K, int64_t rowsP, int64_t Kp, const uint32_t *rowmax,
+                            void *dst, hipStream_t st) {
+  const int64_t total = rowsP * (Kp / 8);
+  if (total <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((total + 255) / 256));
+  hipLaunchKernelGGL(k_split_f16, grid, dim3(256), 0, st, src, sR, sK, rows, K, rowsP, Kp,
+                     rowmax, dst);
+  return hipGetLastError();
+}
+
 hipError_t launch_dense(int S, const DenseParams &p, hipStream_t st) {
+  // S: 0 FP32, 2 bf16x3, 3 bf16x6, 4 scaled f16x3
   const int64_t nblk = S == 0 ? (p.Mp / BM) * (p.Np / BN) : (p.Mp / BMH) * (p.Np / BNH);
   if (nblk <= 0) return hipSuccess;
   if (nblk > 0x7fffffff) return hipErrorInvalidConfiguration;
   const dim3 grid((unsigned)nblk);
   switch (S) {
     case 0: hipLaunchKernelGGL(k_dense_f32, grid, dim3(256), 0, st, p); break;
-    case 2: hipLaunchKernelGGL(k_dense_bf16<2>, grid, dim3(512), 0, st, p); break;
-    case 3: hipLaunchKernelGGL(k_dense_bf16<3>, grid, dim3(512), 0, st, p); break;
+    case 2: hipLaunchKernelGGL((k_dense_bf16<2, false>), grid, dim3(512), 0, st, p); break;
+    case 3: hipLaunchKernelGGL((k_dense_bf16<3, false>), grid, dim3(512), 0, st, p); break;
+    case 4: hipLaunchKernelGGL((k_dense_bf16<2, true>), grid, dim3(512), 0, st, p); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -122,9 +123,90 @@ hipError_t adam_step(hmlMlp_t *n, float lr) {
 
 // ---- selector ------------------------------------------------------------------
 
+// Host copy of the installed selector network.  Selection runs its forward
+// pass on the host (about 13k multiply-adds per new problem), so a GEMM
+// call never allocates, synchronises or launches anything for it.
+struct HostNet {
+  std::vector<int> dims;      // n_in, hidden..., n_out
+  std::vector<float> params;  // per layer W[in][out] then bias[out]
+};
+
 std::mutex g_sel_mutex;
-hmlMlp_t *g_sel_net = nullptr;
+std::shared_ptr<const HostNet> g_sel_host;  // the installed selector
+hmlMlp_t *g_sel_net = nullptr;              // device net it was copied from
 std::unordered_map<std::string, hmlSgemmVariant> g_sel_memo;
+// Decisions memoised per problem key; a process that sees more distinct
+// problems than this starts the memo afresh.
+constexpr size_t kSelMemoCap = 4096;
+
+std::shared_ptr<const HostNet> host_copy(const std::vector<int> &dims,
+                                         std::vector<float> params) {
+  auto h = std::make_shared<HostNet>();
+  h->dims = dims;
+  h->params = std::move(params);
+  return h;
+}
+
+// Logits of one input row: ReLU after every layer but the last, sums in
+// double in input order.
+void host_forward(const HostNet &n, const float *x, double *logits) {
+  std::vector<double> a(x, x + n.dims[0]), z;
+  size_t off = 0;
+  const int L = (int)n.dims.size() - 1;
+  for (int l = 1; l <= L; ++l) {
+    const int I = n.dims[l - 1], O = n.dims[l];
+    const float *W = n.params.data() + off;
+    const float *b = W + (size_t)I * O;
+    z.assign(O, 0.0);
+    for (int o = 0; o < O; ++o) {
+      double acc = b[o];
+      for (int i = 0; i < I; ++i) acc += a[i] * (double)W[(size_t)i * O + o];
+      z[o] = (l < L && acc < 0.0) ? 0.0 : acc;
+    }
+    off += (size_t)I * O + O;
+    a.swap(z);
+  }
+  for (size_t o = 0; o < a.size(); ++o) logits[o] = a[o];
+}
+
+// Reads an MLP file: header {magic "HMLP", n_in, hidden, n_hidden, n_out},
+// then the parameters as floats.
+hmlStatus read_mlp_file(const char *path, int32_t hdr[5], std::vector<float> *params) {
+  FILE *f = fopen(path, "rb");
+  if (!f) return fail(HML_STATUS_IO_ERROR, "cannot open '%s'", path);
+  if (fread(hdr, sizeof(int32_t) * 5, 1, f) != 1 || hdr[0] != 0x504c4d48 || hdr[1] <= 0 ||
+      hdr[2] <= 0 || hdr[3] < 0 || hdr[3] > 64 || hdr[4] <= 1 || hdr[1] > 4096 ||
+      hdr[2] > 4096 || hdr[4] > 4096) {
+    fclose(f);
+    return fail(HML_STATUS_PARSE_ERROR, "'%s' is not an MLP file", path);
+  }
+  int64_t count = 0, in = hdr[1];
+  for (int l = 0; l <= hdr[3]; ++l) {
+    const int64_t out = l < hdr[3] ? hdr[2] : hdr[4];
+    count += in * out + out;
+    in = out;
+  }
+  params->resize((size_t)count);
+  const size_t got = fread(params->data(), sizeof(float), params->size(), f);
+  fclose(f);
+  if (got != params->size()) return fail(HML_STATUS_PARSE_ERROR, "'%s' is truncated", path);
+  return HML_STATUS_SUCCESS;
+}
+
+// Re-reads the parameters of the installed device net (after training or a
+// parameter update).  Caller holds g_sel_mutex.
+void refresh_host_copy_locked(hmlMlp_t *n) {
+  std::vector<float> h((size_t)n->nparam);
+  if (hipStreamSynchronize(n->st) != hipSuccess ||
+      hipMemcpy(h.data(), n->params, sizeof(float) * h.size(), hipMemcpyDeviceToHost) !=
+          hipSuccess) {
+    g_sel_host.reset();
+    g_sel_net = nullptr;
+  } else {
+    g_sel_host = host_copy(n->dims, std::move(h));
+  }
+  g_sel_memo.clear();
+}
 
 int ctmax_for(int64_t N) {
   const int64_t nct = (N + 15) / 16;
@@ -235,54 +317,70 @@ void selector_autoload() {
     FILE *f = fopen(path.c_str(), "rb");
     if (!f) return;
     fclose(f);
-    hmlMlp_t *net = nullptr;
-    if (hmlMlpLoad(&net, path.c_str()) != HML_STATUS_SUCCESS) {
+    int32_t hdr[5];
+    std::vector<float> params;
+    if (read_mlp_file(path.c_str(), hdr, &params) != HML_STATUS_SUCCESS) {
       if (verbosity() >= 1)
         fprintf(stderr, "Warning: selector '%s' not loaded: %s\n", path.c_str(), hmlLastError());
       clear_error();
       return;
     }
-    bool installed = false;
-    {
-      std::lock_guard<std::mutex> lk(g_sel_mutex);
-      if (!g_sel_net && net->n_in == kSelFeatures && net->n_out == kSelClasses) {
-        g_sel_net = net;  // owned by the library until the process ends
-        g_sel_memo.clear();
-        installed = true;
-      }
+    if (hdr[1] != kSelFeatures || hdr[4] != kSelClasses) {
+      if (verbosity() >= 1)
+        fprintf(stderr, "Warning: selector '%s' maps %d features to %d classes, not %d to %d\n",
+                path.c_str(), hdr[1], hdr[4], kSelFeatures, kSelClasses);
+      return;
     }
-    if (!installed) hmlMlpDestroy(net);
+    std::vector<int> dims{hdr[1]};
+    for (int i = 0; i < hdr[3]; ++i) dims.push_back(hdr[2]);
+    dims.push_back(hdr[4]);
+    std::lock_guard<std::mutex> lk(g_sel_mutex);
+    if (!g_sel_host) {  // a net installed by the caller first is kept
+      g_sel_host = host_copy(dims, std::move(params));
+      g_sel_memo.clear();
+    }
   });
 }
 
 bool learned_select_sgemm(const GemmKey &k, hmlSgemmVariant *v) {
   selector_autoload();
-  std::lock_guard<std::mutex> lk(g_sel_mutex);
-  if (!g_sel_net) return false;
+  std::shared_ptr<const HostNet> net;
   char key[160];
   snprintf(key, sizeof key, "%d:%lld:%lld:%lld:%d:%d:%d:%d:%d:%lld:%lld", k.form,
            (long long)k.M, (long long)k.N, (long long)k.K, k.beta0, k.transA,
            k.transB, k.order, (int)k.gemv_ok, (long long)k.cM, (long long)k.cN);
-  auto it = g_sel_memo.find(key);
-  if (it != g_sel_memo.end()) {
-    *v = it->second;
-    return true;
+  {
+    std::lock_guard<std::mutex> lk(g_sel_mutex);
+    if (!g_sel_host) return false;
+    auto it = g_sel_memo.find(key);
+    if (it != g_sel_memo.end()) {
+      *v = it->second;
+      return true;
+    }
+    net = g_sel_host;
   }
-  float x[kSelFeatures], mask[kSelClasses], probs[kSelClasses];
+  float x[kSelFeatures];
+  double logits[kSelClasses];
   selector_features(k, x);
-  int any = 0;
+  host_forward(*net, x, logits);
+  // argmax of the masked softmax = argmax of the allowed logits
+  int best = -1;
+  hmlSgemmVariant pick{};
   for (int c = 0; c < kSelClasses; ++c) {
     hmlSgemmVariant u;
-    mask[c] = selector_variant(k, c, &u) ? 1.f : 0.f;
-    any += mask[c] > 0.f;
+    if (!selector_variant(k, c, &u)) continue;
+    if (best < 0 || logits[c] > logits[best]) {
+      best = c;
+      pick = u;
+    }
   }
-  if (!any) return false;
-  if (hmlMlpPredict(g_sel_net, x, mask, 1, probs) != HML_STATUS_SUCCESS) return false;
-  int best = -1;
-  for (int c = 0; c < kSelClasses; ++c)
-    if (mask[c] > 0.f && (best < 0 || probs[c] > probs[best])) best = c;
-  if (best < 0 || !selector_variant(k, best, v)) return false;
-  g_sel_memo[key] = *v;
+  if (best < 0) return false;
+  *v = pick;
+  std::lock_guard<std::mutex> lk(g_sel_mutex);
+  if (g_sel_host == net) {
+    if (g_sel_memo.size() >= kSelMemoCap) g_sel_memo.clear();
+    g_sel_memo[key] = pick;
+  }
   return true;
 }
 
@@ -338,8 +436,13 @@ hmlStatus hmlMlpCreate(hmlMlp_t **net, int32_t n_in, int32_t hidden,
 hmlStatus hmlMlpDestroy(hmlMlp_t *n) {
   if (!n) return HML_STATUS_SUCCESS;
   {
+    // destroying the installed net uninstalls it
     std::lock_guard<std::mutex> lk(g_sel_mutex);
-    if (g_sel_net == n) { g_sel_net = nullptr; g_sel_memo.clear(); }
+    if (g_sel_net == n) {
+      g_sel_net = nullptr;
+      g_sel_host.reset();
+      g_sel_memo.clear();
+    }
   }
   if (n->st) (void)hipStreamSynchronize(n->st);
   free_batch(n);
@@ -362,11 +465,11 @@ hmlStatus hmlMlpSetParams(hmlMlp_t *n, const float *host, int64_t count) {
   if (!n || !host || count != n->nparam) return fail(HML_STATUS_INVALID_VALUE, "mlp params");
   hipError_t e = hipStreamSynchronize(n->st);
   if (e != hipSuccess) return from_hip(e, "mlp set");
-  {
-    std::lock_guard<std::mutex> lk(g_sel_mutex);
-    if (g_sel_net == n) g_sel_memo.clear();
-  }
-  return from_hip(hipMemcpy(n->params, host, sizeof(float) * count, hipMemcpyHostToDevice), "mlp set");
+  e = hipMemcpy(n->params, host, sizeof(float) * count, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return from_hip(e, "mlp set");
+  std::lock_guard<std::mutex> lk(g_sel_mutex);
+  if (g_sel_net == n) refresh_host_copy_locked(n);
+  return HML_STATUS_SUCCESS;
 }
 
 hmlStatus hmlMlpLossGrad(hmlMlp_t *n, const float *X, const float *T,
@@ -443,7 +546,7 @@ hmlStatus hmlMlpTrain(hmlMlp_t *n, const float *X, const float *T,
   for (void *p : {(void *)dX, (void *)dT, (void *)dM, (void *)dIdx, (void *)dL}) if (p) (void)hipFree(p);
   {
     std::lock_guard<std::mutex> lk(g_sel_mutex);
-    if (g_sel_net == n) g_sel_memo.clear();
+    if (g_sel_net == n) refresh_host_copy_locked(n);
   }
   return from_hip(e, "mlp train");
 }
@@ -486,22 +589,16 @@ hmlStatus hmlMlpSave(hmlMlp_t *n, const char *path) {
 
 hmlStatus hmlMlpLoad(hmlMlp_t **net, const char *path) {
   if (!net || !path) return fail(HML_STATUS_INVALID_VALUE, "mlp load");
-  FILE *f = fopen(path, "rb");
-  if (!f) return fail(HML_STATUS_IO_ERROR, "cannot open '%s'", path);
   int32_t hdr[5];
-  if (fread(hdr, sizeof hdr, 1, f) != 1 || hdr[0] != 0x504c4d48) {
-    fclose(f);
-    return fail(HML_STATUS_PARSE_ERROR, "'%s' is not an MLP file", path);
-  }
-  hmlStatus s = hmlMlpCreate(net, hdr[1], hdr[2], hdr[3], hdr[4], 0);
-  if (s) { fclose(f); return s; }
-  std::vector<float> h((*net)->nparam);
-  const size_t got = fread(h.data(), sizeof(float), h.size(), f);
-  fclose(f);
-  if (got != h.size()) {
+  std::vector<float> h;
+  hmlStatus s = read_mlp_file(path, hdr, &h);
+  if (s) return s;
+  s = hmlMlpCreate(net, hdr[1], hdr[2], hdr[3], hdr[4], 0);
+  if (s) return s;
+  if ((int64_t)h.size() != (*net)->nparam) {
     hmlMlpDestroy(*net);
     *net = nullptr;
-    return fail(HML_STATUS_PARSE_ERROR, "'%s' is truncated", path);
+    return fail(HML_STATUS_PARSE_ERROR, "'%s' has an inconsistent header", path);
   }
   return hmlMlpSetParams(*net, h.data(), (int64_t)h.size());
 }
@@ -549,9 +646,16 @@ hmlStatus hmlSelectorInstall(hmlMlp_t *net) {
   if (net && (net->n_in != kSelFeatures || net->n_out != kSelClasses))
     return fail(HML_STATUS_INVALID_VALUE, "selector net must map %d features to %d classes",
                 kSelFeatures, kSelClasses);
+  selector_autoload();  // so a later first selection cannot replace this choice
   std::lock_guard<std::mutex> lk(g_sel_mutex);
   g_sel_net = net;
   g_sel_memo.clear();
+  if (!net) {
+    g_sel_host.reset();
+    return HML_STATUS_SUCCESS;
+  }
+  refresh_host_copy_locked(net);
+  if (!g_sel_host) return fail(HML_STATUS_HIP_ERROR, "selector install: cannot read the parameters");
   return HML_STATUS_SUCCESS;
 }
 

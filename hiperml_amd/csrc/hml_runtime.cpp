@@ -5,13 +5,15 @@
 // (src/hml_utils.cu:75-82 hmlHandleError, src/hml_sgemm.cu:189-199) and
 // allocates/frees device buffers inside every call (src/hml_utils.cu:147-230).
 // The extended ABI returns status codes instead and keeps one grow-only
-// scratch buffer per device.
+// scratch buffer per (device, stream).
 #include "hml_runtime.h"
 
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <map>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -27,7 +29,6 @@ struct DeviceScratch {
   size_t bytes = 0;
 };
 std::mutex g_ws_mutex;
-std::vector<DeviceScratch> g_ws;
 
 std::mutex g_flush_mutex;
 std::vector<DeviceScratch> g_flush;
@@ -64,30 +65,71 @@ int verbosity() {
   exit(1);
 }
 
-hmlStatus library_workspace(size_t bytes, void **ptr) {
+namespace {
+
+// One workspace slot per (device, stream).  The buffer is shared between
+// the slot and the leases handed out on it; the deleter runs when the last
+// of them lets go, which for a replaced buffer is the end of the last call
+// that was still using it.
+struct WsSlot {
+  std::shared_ptr<void> buf;
+  size_t bytes = 0;
+  uint64_t last_use = 0;
+};
+struct WsKey {
+  int dev;
+  uintptr_t stream;
+  bool operator<(const WsKey &o) const {
+    return dev != o.dev ? dev < o.dev : stream < o.stream;
+  }
+};
+std::map<WsKey, WsSlot> g_ws_slots;
+uint64_t g_ws_clock = 0;
+// Streams come and go; past this many slots the least recently used idle
+// one is dropped (its buffer is freed once no lease holds it).
+constexpr size_t kMaxWsSlots = 16;
+
+void free_device_buffer(void *p) {
+  // hipFree performs an implicit device synchronisation, so every kernel
+  // queued on this buffer has finished before it is released.
+  if (p) (void)hipFree(p);
+}
+
+}  // namespace
+
+hmlStatus library_workspace(size_t bytes, hipStream_t st, WsLease *lease) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return from_hip(e, "hipGetDevice");
+  std::shared_ptr<void> retired;  // released after the mutex
   std::lock_guard<std::mutex> lk(g_ws_mutex);
-  if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
-  DeviceScratch &w = g_ws[dev];
-  if (w.bytes < bytes) {
-    if (w.ptr) {
-      e = hipFree(w.ptr);
-      w.ptr = nullptr;
-      w.bytes = 0;
-      if (e != hipSuccess) return from_hip(e, "hipFree(workspace)");
+  const WsKey key{dev, (uintptr_t)st};
+  auto it = g_ws_slots.find(key);
+  if (it == g_ws_slots.end()) {
+    if (g_ws_slots.size() >= kMaxWsSlots) {
+      auto lru = g_ws_slots.begin();
+      for (auto j = g_ws_slots.begin(); j != g_ws_slots.end(); ++j)
+        if (j->second.last_use < lru->second.last_use) lru = j;
+      retired = std::move(lru->second.buf);
+      g_ws_slots.erase(lru);
     }
-    size_t want = bytes < ((size_t)1 << 20) ? ((size_t)1 << 20) : bytes;
-    e = hipMalloc(&w.ptr, want);
-    if (e != hipSuccess) {
-      w.ptr = nullptr;
-      return fail(HML_STATUS_ALLOC_FAILED, "workspace hipMalloc(%zu): %s",
-                  want, hipGetErrorString(e));
-    }
+    it = g_ws_slots.emplace(key, WsSlot{}).first;
+  }
+  WsSlot &w = it->second;
+  w.last_use = ++g_ws_clock;
+  if (w.bytes < bytes || !w.buf) {
+    const size_t want = bytes < ((size_t)1 << 20) ? ((size_t)1 << 20) : bytes;
+    void *p = nullptr;
+    e = hipMalloc(&p, want);
+    if (e != hipSuccess)
+      return fail(HML_STATUS_ALLOC_FAILED, "workspace hipMalloc(%zu): %s", want,
+                  hipGetErrorString(e));
+    // The old buffer lives on in the leases still holding it.
+    retired = std::move(w.buf);
+    w.buf = std::shared_ptr<void>(p, free_device_buffer);
     w.bytes = want;
   }
-  *ptr = w.ptr;
+  lease->buf = w.buf;
   return HML_STATUS_SUCCESS;
 }
 
@@ -248,6 +290,13 @@ hmlStatus hmlEventSynchronize(void *event) {
 hmlStatus hmlEventElapsedMs(float *ms, void *start, void *stop) {
   return from_hip(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop),
                   "hipEventElapsedTime");
+}
+
+hmlStatus hmlReadProbe(const void *src, size_t bytes, void *sink, void *stream) {
+  if (!src || !sink || bytes % 16 != 0)
+    return fail(HML_STATUS_INVALID_VALUE, "read probe: src, sink and a multiple of 16 bytes");
+  return from_hip(launch_read_probe(src, bytes, (uint32_t *)sink, (hipStream_t)stream),
+                  "read probe");
 }
 
 hmlStatus hmlFlushCaches(void *stream) {

@@ -2,6 +2,7 @@
 // error reporting, per-device workspace, argument helpers.
 #pragma once
 
+#include <memory>
 #include <mutex>
 
 #include <hip/hip_runtime.h>
@@ -20,9 +21,18 @@ hmlStatus fail(hmlStatus s, const char *fmt, ...)
 hmlStatus from_hip(hipError_t e, const char *what);
 void clear_error();
 
-// Library-owned per-device scratch, grown on demand (hipFree synchronises
-// the device, so growth never frees memory a queued kernel still uses).
-hmlStatus library_workspace(size_t bytes, void **ptr);
+// A hold on a library-owned workspace buffer.  The library keeps one
+// grow-only buffer per (device, stream): calls on one stream are ordered by
+// the stream, so they may share it; calls on different streams never do.
+// A buffer replaced by a larger one is freed only when the last lease on it
+// ends (hipFree then waits for the device, so kernels queued under the lease
+// have finished).  Keep the lease alive until the kernels that use ptr() are
+// queued.
+struct WsLease {
+  std::shared_ptr<void> buf;
+  void *ptr() const { return buf.get(); }
+};
+hmlStatus library_workspace(size_t bytes, hipStream_t st, WsLease *lease);
 
 // Splits a workspace into 256-byte aligned pieces.
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }

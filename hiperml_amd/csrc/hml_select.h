@@ -53,13 +53,14 @@ int ilog2_floor(int64_t x);
 
 void heuristic_sgemm(const GemmKey &k, hmlSgemmVariant *v);
 
-// Dense families (plain FP32 / bf16x3 / bf16x6 MFMA tiles).
+// Dense families (plain FP32 / bf16x3 / bf16x6 / f16x3 MFMA tiles).
 inline bool is_dense_family(int f) {
   return f == HML_SGEMM_FAMILY_DENSE || f == HML_SGEMM_FAMILY_BF16X3 ||
-         f == HML_SGEMM_FAMILY_BF16X6;
+         f == HML_SGEMM_FAMILY_BF16X6 || f == HML_SGEMM_FAMILY_F16X3;
 }
 inline int dense_precision(int f) {
-  return f == HML_SGEMM_FAMILY_BF16X3 ? 1 : f == HML_SGEMM_FAMILY_BF16X6 ? 2 : 0;
+  return f == HML_SGEMM_FAMILY_BF16X3 ? 1 : f == HML_SGEMM_FAMILY_BF16X6 ? 2
+         : f == HML_SGEMM_FAMILY_F16X3 ? 3 : 0;
 }
 size_t dense_workspace(int prec, int64_t M, int64_t N, int64_t K);
 hmlStatus run_dense(int prec, const float *A, int64_t sAm, int64_t sAk,

@@ -57,8 +57,10 @@ typedef enum {
   HML_SGEMM_FAMILY_BF16X6 = 6, /* split-precision bf16 MFMA, 3-term split  */
   HML_SGEMM_FAMILY_DOTB   = 7, /* MFMA f32, K-contiguous A, B shared by the
                                   workgroup through LDS                   */
-  HML_SGEMM_FAMILY_GEMVN  = 8  /* N in {2, 4, 8}, K-contiguous A: the GEMV
+  HML_SGEMM_FAMILY_GEMVN  = 8, /* N in {2, 4, 8}, K-contiguous A: the GEMV
                                   DOT kernels with N accumulators per row */
+  HML_SGEMM_FAMILY_F16X3  = 9  /* split-precision fp16 MFMA, 2-term split
+                                  with power-of-two row / column scales   */
 } hmlSgemmFamily;
 
 /* Skinny variants (DOT, AXPY): a wave computes rt row tiles (16 rows DOT,
@@ -106,10 +108,13 @@ typedef struct {
 /* ---- GEMM / GEMV on device pointers --------------------------------------
  * C = alpha * op(A) * op(B) + beta * C, C is M x N, op(A) M x K, op(B) K x N,
  * in `order` with leading dimensions lda/ldb/ldc (BLAS convention).
- * workspace may be NULL (the library then uses a per-device buffer it owns
- * and grows on demand; such calls must not be captured into a graph the
- * first time a larger buffer is needed); otherwise ws_bytes must be at least
- * hmlSgemmWorkspaceSize() for the same arguments. */
+ * workspace may be NULL: the library then uses a buffer it owns per (device,
+ * stream), so calls on different streams never share one; it grows on
+ * demand with hipMalloc, so such calls must not be captured into a graph
+ * the first time a stream needs a larger buffer.  Otherwise ws_bytes must be
+ * at least hmlSgemmWorkspaceSize() for the same arguments (for strides past
+ * the skinny kernels' 31-bit offsets the call needs the dense kernel's
+ * operand copies instead and says how many bytes in its error). */
 hmlStatus hmlSgemmEx(hmlOrder order, hmlOp transA, hmlOp transB,
                      int64_t M, int64_t N, int64_t K, float alpha,
                      const float *A, int64_t lda, const float *B, int64_t ldb,
@@ -144,8 +149,14 @@ hmlStatus hmlSgemmSelect(hmlOrder order, hmlOp transA, hmlOp transB,
  * (v_mfma_f32_16x16x4f32), 1 = bf16x3 (FP32 inputs split into hi/lo bf16,
  * products hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16, FP32
  * accumulation), 2 = bf16x6 (hi/mid/lo split, the six products down to
- * 2^-16 relative weight).  Other arguments as hmlSgemmEx; the workspace holds
- * the split planes (hmlSgemmDenseWorkspaceSize). */
+ * 2^-16 relative weight), 3 = f16x3 (rows of A and columns of B scaled by
+ * powers of two so their largest magnitude is in [2^14, 2^15), then split
+ * into hi/lo fp16, products hi*hi and hi*lo + lo*hi accumulated separately
+ * on v_mfma_f32_16x16x32_f16, the scales removed exactly in the epilogue;
+ * elements below about 2^-24 of their row's largest lose precision, and a
+ * row or column holding Inf or NaN is not scaled).  Other arguments as
+ * hmlSgemmEx; the workspace holds the split planes
+ * (hmlSgemmDenseWorkspaceSize). */
 hmlStatus hmlSgemmDense(int32_t precision, hmlOrder order, hmlOp transA,
                         hmlOp transB, int64_t M, int64_t N, int64_t K,
                         float alpha, const float *A, int64_t lda,
@@ -252,6 +263,19 @@ hmlStatus hmlSgemvAutotune(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
                            int32_t trials, int32_t flush, uint64_t seed,
                            hmlSgemvVariant *best, double *best_ms,
                            double *heuristic_ms, const char *db_path);
+/* Summary tables of a timing database, the counterpart of the reference's
+ * hmlSgemmKernelConfigPrintStats (src/hml_sgemm_kernel.cu:318-354) and of
+ * the tables its SGEMV sweep prints (src/hml_sgemv_kernel.cu:391-632): per
+ * layout group and variant family, the best GFLOP/s for every (K, N) (SGEMV:
+ * (M, N)) and the max / min spread of the family's valid variants in
+ * percent, then the overall winner per cell.  kind 0 = SGEMM rows, 1 =
+ * SGEMV rows; out_path NULL writes to stderr.  hmlSgemmAutotune /
+ * hmlSgemvAutotune skip candidates that db_path already records for the
+ * shape (an interrupted sweep resumes), and hmlSgemmKernelConfig(NULL) /
+ * hmlSgemvKernelConfig(NULL) sweep with $HML_AUTOTUNE_DB as their database
+ * when it is set (printing these tables at HML_VERBOSE >= 2). */
+hmlStatus hmlAutotuneStats(const char *db_path, int32_t kind,
+                           const char *out_path);
 /* Install (or replace) the v2 rule of the shape's bucket. */
 hmlStatus hmlSgemmConfigAddRule(const hmlGemmShape *shape,
                                 const hmlSgemmVariant *variant);
@@ -350,6 +374,10 @@ hmlStatus hmlEventDestroy(void *event);
 hmlStatus hmlEventRecord(void *event, void *stream);
 hmlStatus hmlEventSynchronize(void *event);
 hmlStatus hmlEventElapsedMs(float *ms, void *start, void *stop);
+/* Read-bandwidth probe for benchmarks: streams `bytes` (a multiple of 16)
+ * of device memory with 16-byte non-temporal loads and writes nothing (sink,
+ * 4 device bytes, is written only in a case that does not occur). */
+hmlStatus hmlReadProbe(const void *src, size_t bytes, void *sink, void *stream);
 /* Writes `bytes` of junk through the caches so the next timed kernel sees
  * cold L2/MALL (used by the autotuner and benchmarks). */
 hmlStatus hmlFlushCaches(void *stream);

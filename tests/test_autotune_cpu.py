@@ -107,3 +107,77 @@ def test_autotune_rejects_bad_shape():
     with FakeTimer(lambda k, d, v: 1.0):
         st, *_ = sgemm_autotune(s)
     assert st == 1
+
+
+def test_resume_skips_recorded_candidates(tmp_path):
+    """The timing database is the checkpoint: a second run over the same
+    shape measures nothing and returns the recorded winner; a run that was
+    cut short measures only the missing candidates."""
+    s = hml.GemmShape(8192, 16, 8192, 0, 0, 0, 1)
+    cands = candidates(s)
+    cost = lambda k, d, v: 1.0 + 0.01 * sum(v)
+    db = str(tmp_path / "db.csv")
+    with FakeTimer(cost) as ft:
+        st, best, bt, ht = sgemm_autotune(s, db)
+    assert st == 0 and len(ft.calls) == len(cands)
+    with FakeTimer(lambda k, d, v: pytest.fail("re-measured %s" % (v,))) as ft:
+        st2, best2, bt2, ht2 = sgemm_autotune(s, db)
+    assert st2 == 0 and ft.calls == []
+    assert best2.astuple() == best.astuple() and bt2 == pytest.approx(bt)
+    assert ht2 == pytest.approx(ht)
+    assert len(list(csv.DictReader(open(db)))) == len(cands)
+    # an interrupted sweep: keep the header, the first 5 rows and half a row
+    lines = open(db).read().splitlines(True)
+    db2 = str(tmp_path / "db2.csv")
+    open(db2, "w").write("".join(lines[:6]) + lines[6][:10])
+    with FakeTimer(cost) as ft:
+        st3, best3, bt3, _ = sgemm_autotune(s, db2)
+    assert st3 == 0 and len(ft.calls) == len(cands) - 5
+    assert best3.astuple() == best.astuple()
+    # a different shape in the same database is measured in full
+    s2 = hml.GemmShape(4096, 16, 8192, 0, 0, 0, 1)
+    with FakeTimer(cost) as ft:
+        assert sgemm_autotune(s2, db)[0] == 0
+    assert len(ft.calls) == len(candidates(s2))
+
+
+def test_sgemv_resume(tmp_path):
+    db = str(tmp_path / "v.csv")
+    best = hml.SgemvVariant()
+    bt, ht = ctypes.c_double(), ctypes.c_double()
+    for expect_calls in (True, False):
+        with FakeTimer(lambda k, d, v: 1.0 + v[1] * 0.01) as ft:
+            st = hml.lib.hmlSgemvAutotune(0, 0, 4096, 4096, 3, 0, 1, ctypes.byref(best),
+                                          ctypes.byref(bt), ctypes.byref(ht), db.encode())
+        assert st == 0
+        assert bool(ft.calls) == expect_calls
+
+
+def test_stats_tables(tmp_path):
+    """Summary tables of the timing database (the counterpart of the
+    reference's GFLOPS and max / min tables): best GFLOP/s per family for
+    every (K, N) and the max / min spread."""
+    db = str(tmp_path / "db.csv")
+    # cost in ms: dot variants 1 ms .. 2 ms (spread 200%), others slower
+    def cost(kind, d, v):
+        base = 1.0 if v[0] == 1 else 4.0
+        return base * (1.0 + (v[3] > 1))
+    with FakeTimer(cost):
+        for N in (16, 32):
+            for K in (4096, 8192):
+                assert sgemm_autotune(hml.GemmShape(8192, N, K, 0, 0, 0, 1), db)[0] == 0
+    out = str(tmp_path / "stats.txt")
+    assert hml.lib.hmlAutotuneStats(db.encode(), 0, out.encode()) == 0
+    text = open(out).read()
+    assert "dot GFLOPS [M=8192 order=0 transA=0 transB=0 beta0=1]" in text
+    assert "dot max / min" in text and "best GFLOPS (family)" in text
+    lines = text.splitlines()
+    i = lines.index([l for l in lines if l.startswith("dot GFLOPS")][0])
+    assert lines[i + 1].split() == ["K", "N=16", "N=32"]
+    row = lines[i + 3].split()
+    assert row[0] == "4096"
+    # best dot time 1 ms: 2 * 8192 * 16 * 4096 / 1e6 GFLOP/s
+    assert float(row[1]) == pytest.approx(2 * 8192 * 16 * 4096 / 1e6, rel=1e-3)
+    j = lines.index("dot max / min [M=8192 order=0 transA=0 transB=0 beta0=1]:")
+    assert lines[j + 3].split()[1] == "200%"
+    assert hml.lib.hmlAutotuneStats(b"/nonexistent/db.csv", 0, None) == 5

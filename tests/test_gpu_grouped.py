@@ -104,3 +104,42 @@ def test_grouped_rejects_wide_problem(hml):
     with pytest.raises(hml.HmlError) as e:
         hml.GroupedPlan([(16, 64, 32, d, d, d)])
     assert e.value.status == 6
+
+
+def test_grouped_baseline_suite_vs_fp64(hml):
+    """The benchmark's batched workload at full size: 512 problems, M and K
+    log-uniform in [64, 65536] and [64, 4096], N uniform in 1..64 (fixed
+    seed), uniform[-1,1] entries filled on the device.  Run directly and from
+    the captured graph, every problem is checked element by element by the
+    FP64 kernel against 2 gamma_{K+2} |A||B| (the grouped tile runs its whole
+    K in one FP32 accumulation), and both runs agree bit for bit."""
+    from bench import grouped_suite_shapes
+    shapes = grouped_suite_shapes(512, seed=0)
+    assert max(m for m, n, k in shapes) > 30000 and max(k for m, n, k in shapes) > 3000
+    probs = []
+    for i, (M, N, K) in enumerate(shapes):
+        dA, dB, dC = (hml.DeviceBuffer(4 * M * K), hml.DeviceBuffer(4 * K * N),
+                      hml.DeviceBuffer(4 * M * N))
+        dA.fill_uniform(M * K, 3 * i + 1)
+        dB.fill_uniform(K * N, 3 * i + 2)
+        probs.append((M, N, K, dA, dB, dC))
+    plan = hml.GroupedPlan(probs, 1.0, 0.0)
+    first = []
+    for graph in (False, True):
+        print("grouped suite, graph=%s" % graph, flush=True)
+        for M, N, K, dA, dB, dC in probs:  # NaN everywhere: every output rewritten
+            hml.check(hml.lib.hmlMemsetD8(dC.ptr, 0xFF, 4 * M * N))
+        plan.run(graph=graph)
+        hml.synchronize()
+        worst = 0.0
+        for j, (M, N, K, dA, dB, dC) in enumerate(probs):
+            st = hml.check_f64(M, N, K, dA, dB, dC)
+            assert st["nonfinite"] == 0, (j, M, N, K)
+            assert st["max_rel_bound"] <= 2 * hml.gamma(K + 2), (j, M, N, K, st)
+            worst = max(worst, st["max_rel_bound"] / hml.gamma(K + 2))
+            if not graph:
+                first.append(dC.to_numpy((M, N)) if M * N <= 1 << 20 else None)
+            elif first[j] is not None:
+                np.testing.assert_array_equal(dC.to_numpy((M, N)), first[j])
+        print("worst max_err/bound in units of gamma_K+2: %.3f" % worst)
+    plan.destroy()
