@@ -50,10 +50,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time budget of the baseline sample")
+    ap.add_argument("--quick", action="store_true",
+                    help="only the shape sweep: leave out the reference-shape, dense "
+                         "split-precision and grouped suites of the JSON line")
     ap.add_argument("--dense", action="store_true",
-                    help="also run the split-precision dense suite (extra JSON line)")
+                    help="also print the split-precision dense suite as a JSON line of its own")
     ap.add_argument("--grouped", action="store_true",
-                    help="also run the grouped 512-problem suite (extra JSON line)")
+                    help="also print the grouped 512-problem suite as a JSON line of its own")
     ap.add_argument("--host", action="store_true",
                     help="also time the host-pointer entry points (PCIe copies included)")
     ap.add_argument("--kmeans", action="store_true",
@@ -83,9 +86,9 @@ def main():
     hml.check(hml.lib.hmlInit())
     if args.only_extra:
         if args.dense:
-            bench_dense(hml)
+            print(json.dumps(bench_dense(hml)))
         if args.grouped:
-            bench_grouped(hml)
+            print(json.dumps(bench_grouped(hml)))
         if args.kmeans:
             bench_kmeans(hml, cpu=not args.no_cpu)
         if args.host:
@@ -219,11 +222,22 @@ def main():
         except Exception:
             pass
 
-    probe = bandwidth_probe(hml, bufs["gemmA_0"], S)
+    probe = bandwidth_probe(hml, bufs["gemmA_0"], S,
+                            read_src=[bufs["gemvA_16384_16384_0"], bufs["gemvA_16384_16384_1"]])
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(hml, calls, args.cpu_seconds)
+
+    extras = {}
+    if not args.quick:
+        # the other workloads, measured after the timed steps
+        extras["reference_shape_2048x16x2048"] = bench_reference_shape(hml, cpu=not args.no_cpu)
+        for b in list(bufs.values()):
+            b.free()
+        bufs.clear()
+        extras["dense_split_precision"] = bench_dense(hml)
+        extras["grouped_512"] = bench_grouped(hml)
 
     out = {
         "metric": "SGEMM/SGEMV GFLOP/s + HBM GB/s vs plain FP32 at matched FP64-oracle error",
@@ -249,6 +263,7 @@ def main():
         "cpu_baseline": cpu,
         "shapes": shapes,
     }
+    out.update(extras)
     print(json.dumps(out))
     sys.stdout.flush()
     if args.calibrate_copy:
@@ -261,20 +276,36 @@ def main():
             dst.fill_uniform(n // 4, 99 + i, stream=S)
         stream.synchronize()
         print("calibration: 3 copies + 3 fills of %d bytes" % n, file=sys.stderr)
-    if args.dense:
-        bench_dense(hml)
-    if args.grouped:
-        bench_grouped(hml)
+    if args.dense and args.quick:
+        print(json.dumps(bench_dense(hml)))
+    if args.grouped and args.quick:
+        print(json.dumps(bench_grouped(hml)))
     if args.kmeans:
         bench_kmeans(hml, cpu=not args.no_cpu)
     if args.host:
         bench_host(hml)
 
 
-def bandwidth_probe(hml, src, S, nbytes=256 << 20, reps=5):
-    """Achievable HBM bandwidth beside the vendor peak: a device-to-device
-    copy of 256 MiB (hipMemcpy's copy kernel: reads and writes nbytes) and
-    the library's fill kernel over 256 MiB (writes only), HIP-event timed."""
+def bandwidth_probe(hml, src, S, nbytes=256 << 20, reps=5, read_src=None):
+    """Achievable HBM bandwidth beside the vendor peak: a read-only stream
+    over read_src (hmlReadProbe: 16-byte non-temporal loads, no writes; 1 GiB
+    alternating between two buffers so the 256 MB MALL holds neither), a
+    device-to-device copy of 256 MiB (hipMemcpy's copy kernel: reads and
+    writes nbytes) and the library's fill kernel over 256 MiB (writes only),
+    HIP-event timed."""
+    read = None
+    if read_src:
+        sink = hml.DeviceBuffer(256)
+        rb = read_src[0].nbytes // 16 * 16
+        for b in read_src:
+            hml.check(hml.lib.hmlReadProbe(b.ptr, rb, sink.ptr, S))
+        e0, e1 = hml.Event(), hml.Event()
+        e0.record(S)
+        for i in range(2 * reps):
+            hml.check(hml.lib.hmlReadProbe(read_src[i % len(read_src)].ptr, rb, sink.ptr, S))
+        e1.record(S)
+        e1.synchronize()
+        read = 2.0 * reps * rb / (e0.elapsed_ms(e1) * 1e-3) / 1e9
     dst = hml.DeviceBuffer(nbytes)
     hml.check(hml.lib.hmlMemcpyDtoD(dst.ptr, src.ptr, nbytes))
     hml.synchronize()
@@ -292,20 +323,91 @@ def bandwidth_probe(hml, src, S, nbytes=256 << 20, reps=5):
     e1.synchronize()
     write = 1.0 * nbytes * reps / (e0.elapsed_ms(e1) * 1e-3) / 1e9
     dst.free()
-    return {"copy_read_plus_write_GB_s": round(copy, 1), "fill_write_GB_s": round(write, 1),
-            "bytes": nbytes, "frac_of_peak": {"copy": round(copy / HBM_PEAK_GBS, 4),
-                                             "fill": round(write / HBM_PEAK_GBS, 4)}}
+    out = {"copy_read_plus_write_GB_s": round(copy, 1), "fill_write_GB_s": round(write, 1),
+           "bytes": nbytes, "frac_of_peak": {"copy": round(copy / HBM_PEAK_GBS, 4),
+                                            "fill": round(write / HBM_PEAK_GBS, 4)}}
+    if read is not None:
+        out["read_only_GB_s"] = round(read, 1)
+        out["read_only_bytes"] = read_src[0].nbytes // 16 * 16
+        out["frac_of_peak"]["read_only"] = round(read / HBM_PEAK_GBS, 4)
+    return out
+
+
+def bench_reference_shape(hml, cpu=True, reps=20):
+    """The reference app's plain FP32 case: row-major C = 1.0 A B + 0.0 C,
+    M = K = 2048, N = 16, A and B uniform[-1,1] from a fixed seed, C
+    prefilled with finite values.  GPU: hmlSgemmEx on device buffers, HIP
+    events, each call after a cache flush (cold: the 17 MB of operands fit
+    the 256 MB MALL) and back to back (warm).  CPU: the restated reference
+    CPU path (one thread) on the full problem.  Both against the FP64 check
+    kernel, plus the reference app's checksum test (relative difference of
+    the sums of C, apps/gemm/hml_sgemm_main.cu:294-305,497-511)."""
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle"))
+    M, N, K = 2048, 16, 2048
+    rng = np.random.default_rng(2048)
+    A = rng.uniform(-1, 1, (M, K)).astype(np.float32)
+    B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    C0 = rng.uniform(-1, 1, (M, N)).astype(np.float32)
+    dA, dB = hml.DeviceBuffer.from_numpy(A), hml.DeviceBuffer.from_numpy(B)
+    dC = hml.DeviceBuffer.from_numpy(C0)
+    S = hml.Stream()
+    run = lambda: hml.sgemm_ex(M, N, K, dA, dB, dC, 1.0, 0.0, stream=S.ptr)
+    run()
+    S.synchronize()
+    cold = []
+    e0, e1 = hml.Event(), hml.Event()
+    for _ in range(reps):
+        hml.flush_caches(S.ptr)
+        e0.record(S.ptr)
+        run()
+        e1.record(S.ptr)
+        e1.synchronize()
+        cold.append(e0.elapsed_ms(e1))
+    e0.record(S.ptr)
+    for _ in range(reps):
+        run()
+    e1.record(S.ptr)
+    e1.synchronize()
+    warm = e0.elapsed_ms(e1) / reps
+    cold_ms = float(np.median(cold))
+    nbytes = 4.0 * (M * K + K * N + M * N)
+    gpu = dC.to_numpy((M, N))
+    st = hml.check_f64(M, N, K, dA, dB, dC)
+    v, src = hml.sgemm_select(M, N, K)
+    out = {"workload": "row-major SGEMM C = A B, M = K = 2048, N = 16, uniform[-1,1] (seed 2048)",
+           "variant": list(v.astuple()), "gpu_cold_ms": round(cold_ms, 5),
+           "gpu_warm_ms": round(warm, 5),
+           "gpu_cold_GB_s": round(nbytes / cold_ms / 1e6, 1),
+           "gpu_cold_frac_of_hbm_roofline": round(nbytes / cold_ms / 1e6 / HBM_PEAK_GBS, 4),
+           "gpu_GFLOP_s_cold": round(2.0 * M * N * K / cold_ms / 1e6, 1),
+           "gpu_rel_fro_err_fp64": st["rel_fro"]}
+    if cpu:
+        import oracle
+        Cc = C0.copy()
+        t0 = time.perf_counter()
+        oracle.sgemm_cpu(Cc, A, B, 1.0, 0.0)
+        dt = time.perf_counter() - t0
+        R = A.astype(np.float64) @ B.astype(np.float64)
+        gs, cs = float(np.sum(gpu, dtype=np.float64)), float(np.sum(Cc, dtype=np.float64))
+        out.update({"cpu_path_ms_1thread": round(dt * 1e3, 2),
+                    "cpu_path_GB_s": round(nbytes / dt / 1e9, 3),
+                    "cpu_path_rel_fro_err_fp64": float(np.linalg.norm(Cc - R) / np.linalg.norm(R)),
+                    "csum_rel_diff_gpu_vs_cpu_path": abs(gs - cs) / abs(cs),
+                    "gpu_speedup_vs_cpu_path": round(dt * 1e3 / cold_ms, 1)})
+    return out
 
 
 def bench_dense(hml, reps=5):
-    """Split-precision bf16x3 / bf16x6 vs plain FP32 on the matrix cores:
-    M = N = K = 8192 and M = N = 8192, K = 256, inputs uniform[-1,1] (set A)
-    and log-uniform magnitudes in [1e-3, 1e3] with random sign (set B);
-    error against the FP64 check kernel."""
+    """Split-precision bf16x3 / bf16x6 / scaled f16x3 vs plain FP32 on the
+    matrix cores: M = N = K = 8192 and M = N = 8192, K = 256, inputs
+    uniform[-1,1] (set A) and log-uniform magnitudes in [1e-3, 1e3] with
+    random sign (set B); error against the FP64 check kernel.  Each call
+    includes its split pass (the split planes are part of the work)."""
     import numpy as np
     stream = hml.Stream()  # keep the object: its destructor destroys the stream
     S = stream.ptr
     rows = []
+    precisions = ("fp32", "bf16x3", "bf16x6", "f16x3")
     for (M, N, K) in ((8192, 8192, 8192), (8192, 8192, 256)):
         rng = np.random.default_rng(M + K)
         for dist in ("uniform", "loguniform"):
@@ -318,7 +420,7 @@ def bench_dense(hml, reps=5):
             dA, dB = hml.DeviceBuffer.from_numpy(A), hml.DeviceBuffer.from_numpy(B)
             dC = hml.DeviceBuffer(4 * M * N)
             del A, B
-            for prec in ("fp32", "bf16x3", "bf16x6"):
+            for prec in precisions:
                 ws = hml.lib.hmlSgemmDenseWorkspaceSize(hml.PRECISION[prec], M, N, K)
                 W = hml.DeviceBuffer(ws)
                 run = lambda: hml.sgemm_dense(M, N, K, dA, dB, dC, precision=prec,
@@ -343,9 +445,32 @@ def bench_dense(hml, reps=5):
                 q["inputs"] == r["inputs"] and q["precision"] == "fp32"][0]
         r["speedup_vs_fp32"] = round(base["ms"] / r["ms"], 3)
         r["err_vs_fp32"] = round(r["rel_fro_err_fp64"] / base["rel_fro_err_fp64"], 3)
-    print(json.dumps({"suite": "dense split-precision (bf16x3/bf16x6) vs plain FP32 MFMA",
-                      "peaks": {"fp32_TFLOP_s": FP32_PEAK_TFLOPS, "bf16_dense_TFLOP_s": 2516.6},
-                      "rows": rows}))
+    # matched error: the fastest split whose error is no worse than FP32's on
+    # every shape and input set, and its smallest speedup over those cases
+    summary = {}
+    for prec in precisions[1:]:
+        mine = [r for r in rows if r["precision"] == prec]
+        summary[prec] = {"max_err_vs_fp32": max(r["err_vs_fp32"] for r in mine),
+                         "min_speedup_vs_fp32": min(r["speedup_vs_fp32"] for r in mine),
+                         "matched_error": all(r["err_vs_fp32"] <= 1.0 for r in mine)}
+    return {"suite": "dense split precision vs plain FP32 MFMA (8192^3 and 8192x8192x256, "
+                     "uniform and log-uniform inputs, error vs the FP64 check kernel)",
+            "peaks": {"fp32_TFLOP_s": FP32_PEAK_TFLOPS, "bf16_fp16_dense_TFLOP_s": 2516.6},
+            "summary": summary, "rows": rows}
+
+
+def grouped_suite_shapes(count=512, seed=0):
+    """The batched workload: `count` row-major problems (M, N, K) with M and
+    K log-uniform in [64, 65536] and [64, 4096] (rounded), N uniform in
+    1..64, drawn in that order from numpy's default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        M = int(round(np.exp(rng.uniform(np.log(64), np.log(65536)))))
+        K = int(round(np.exp(rng.uniform(np.log(64), np.log(4096)))))
+        N = int(rng.integers(1, 65))
+        out.append((M, N, K))
+    return out
 
 
 def bench_grouped(hml, count=512, reps=5, seed=0):
@@ -353,13 +478,9 @@ def bench_grouped(hml, count=512, reps=5, seed=0):
     [64,4096], N uniform in 1..64, uniform[-1,1]) in one persistent grouped
     launch (direct and hipGraph) vs 512 sequential hmlSgemmEx calls."""
     import numpy as np
-    rng = np.random.default_rng(seed)
     S = hml.Stream()
     probs, nbytes, flops = [], 0.0, 0.0
-    for i in range(count):
-        M = int(round(np.exp(rng.uniform(np.log(64), np.log(65536)))))
-        K = int(round(np.exp(rng.uniform(np.log(64), np.log(4096)))))
-        N = int(rng.integers(1, 65))
+    for i, (M, N, K) in enumerate(grouped_suite_shapes(count, seed)):
         A = hml.DeviceBuffer(4 * M * K); A.fill_uniform(M * K, 3 * i + 1, stream=S.ptr)
         B = hml.DeviceBuffer(4 * K * N); B.fill_uniform(K * N, 3 * i + 2, stream=S.ptr)
         C = hml.DeviceBuffer(4 * M * N)
@@ -393,16 +514,28 @@ def bench_grouped(hml, count=512, reps=5, seed=0):
     worst = 0.0
     for M, N, K, A, B, C in probs[:64]:
         st = hml.check_f64(M, N, K, A, B, C)
-        worst = max(worst, st["max_rel_bound"])
+        worst = max(worst, st["max_rel_bound"] / hml.gamma(K + 2))
     plan.destroy()
-    print(json.dumps({"suite": "grouped skinny SGEMM, %d problems" % count,
-                      "algorithmic_GB": round(nbytes / 1e9, 3), "GFLOP": round(flops / 1e9, 1),
-                      "sequential_ms": round(t_seq, 4), "grouped_ms": round(t_dir, 4),
-                      "grouped_graph_ms": round(t_graph, 4),
-                      "speedup_graph_vs_sequential": round(t_seq / t_graph, 3),
-                      "grouped_graph_GB_s": round(nbytes / t_graph / 1e6, 1),
-                      "tiles": info["tiles"], "blocks": info["blocks"],
-                      "max_err_over_bound_first64": worst}))
+    # roofline of the suite: per problem the longer of its HBM time (each
+    # operand once) and its FP32 time, summed
+    floor_ms = sum(max(4.0 * (M * K + K * N + M * N) / (HBM_PEAK_GBS * 1e9),
+                       2.0 * M * N * K / (FP32_PEAK_TFLOPS * 1e12))
+                   for M, N, K, *_ in probs) * 1e3
+    out = {"suite": "grouped skinny SGEMM, %d problems (M log-uniform [64,65536], K log-uniform "
+                    "[64,4096], N uniform 1..64, seed %d)" % (count, seed),
+           "algorithmic_GB": round(nbytes / 1e9, 3), "GFLOP": round(flops / 1e9, 1),
+           "sequential_ms": round(t_seq, 4), "grouped_ms": round(t_dir, 4),
+           "grouped_graph_ms": round(t_graph, 4),
+           "speedup_graph_vs_sequential": round(t_seq / t_graph, 3),
+           "grouped_graph_GB_s": round(nbytes / t_graph / 1e6, 1),
+           "roofline_floor_ms": round(floor_ms, 4),
+           "frac_of_roofline": round(floor_ms / t_graph, 4),
+           "tiles": info["tiles"], "blocks": info["blocks"],
+           "max_err_over_gammaK_first64": round(worst, 4)}
+    for p in probs:
+        for b in p[3:]:
+            b.free()
+    return out
 
 
 def bench_kmeans(hml, reps=5, rows=1 << 24, cpu=True):
