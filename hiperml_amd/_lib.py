@@ -74,6 +74,10 @@ _SIGS = {
                            ctypes.c_float, c_void_p, c_i64, c_void_p, c_i64,
                            ctypes.c_float, c_void_p, c_i64, c_u32, c_void_p,
                            c_size, c_void_p]),
+    "hmlSgemmExNoAlloc": (c_i32, [c_i32, c_i32, c_i32, c_i64, c_i64, c_i64,
+                                  ctypes.c_float, c_void_p, c_i64, c_void_p,
+                                  c_i64, ctypes.c_float, c_void_p, c_i64,
+                                  c_u32, c_void_p]),
     "hmlSgemmExVariant": (c_i32, [c_i32, c_i32, c_i32, c_i64, c_i64, c_i64,
                                   ctypes.c_float, c_void_p, c_i64, c_void_p,
                                   c_i64, ctypes.c_float, c_void_p, c_i64,

@@ -203,7 +203,9 @@ hmlStatus hmlInit(void) {
   hipError_t e = hipGetDeviceCount(&n);
   if (e != hipSuccess) return from_hip(e, "hipGetDeviceCount");
   if (n <= 0) return fail(HML_STATUS_HIP_ERROR, "no GPU device");
-  return HML_STATUS_SUCCESS;
+  // set-up of hmlSgemmExNoAlloc's fixed scratch on the current device
+  void *scratch = nullptr;
+  return noalloc_scratch(&scratch);
 }
 
 hmlStatus hmlSetDevice(int device) {

@@ -34,6 +34,10 @@ struct WsLease {
 };
 hmlStatus library_workspace(size_t bytes, hipStream_t st, WsLease *lease);
 
+// Device address of the current device's fixed scratch array of
+// hmlSgemmExNoAlloc (resolved once per device).
+hmlStatus noalloc_scratch(void **addr);
+
 // Splits a workspace into 256-byte aligned pieces.
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 

@@ -121,6 +121,23 @@ hmlStatus hmlSgemmEx(hmlOrder order, hmlOp transA, hmlOp transB,
                      float beta, float *C, int64_t ldc, uint32_t flags,
                      void *workspace, size_t ws_bytes, void *stream);
 
+/* hmlSgemmEx for callers that cannot allocate device memory (code under
+ * stream capture, serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference, whose every call allocates
+ * (src/hml_sgemm.cu:155-210).  Partial sums go to a fixed device array of
+ * HML_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends, instead of
+ * a workspace; the call allocates and frees nothing.  A problem whose
+ * variant needs more scratch than that returns
+ * HML_STATUS_WORKSPACE_TOO_SMALL.  All calls on one GPU share its array:
+ * calls that overlap in time on one GPU race (order them on one stream).
+ * Other arguments, selection and results as hmlSgemmEx. */
+#define HML_NOALLOC_SCRATCH_BYTES ((size_t)16 << 20)
+hmlStatus hmlSgemmExNoAlloc(hmlOrder order, hmlOp transA, hmlOp transB,
+                            int64_t M, int64_t N, int64_t K, float alpha,
+                            const float *A, int64_t lda, const float *B,
+                            int64_t ldb, float beta, float *C, int64_t ldc,
+                            uint32_t flags, void *stream);
+
 /* Same with an explicit variant (autotuner, tests).  variant->family must
  * match the layout (DOT needs op(A) contiguous along K after orientation,
  * AXPY contiguous along M); a mismatched or invalid variant returns
