@@ -153,25 +153,29 @@ __global__ __launch_bounds__(256) void k_compare(const float *C,
   if (t < 5) stats[(int64_t)blockIdx.x * 5 + t] = red[t][0];
 }
 
-// Read-only stream: every thread reads 16-byte chunks (4 independent loads
-// per step, grid-stride) and folds them with XOR; the fold is stored only in
-// the never-taken case that it equals a fixed odd constant, so the kernel
-// moves no bytes except the reads.
+// Read-only stream: each wave reads 8 KB contiguous pieces (8 loads of
+// 1 KB, 16 bytes per lane, all in flight together), pieces grid-strided, and
+// folds them with XOR; the fold is stored only in the never-taken case that
+// it equals a fixed odd constant, so the kernel moves no bytes except the
+// reads.
 typedef uint32_t probe_u4 __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256) void k_read_probe(const probe_u4 *__restrict__ src,
                                                     size_t n16, uint32_t *sink) {
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const size_t nw = (size_t)gridDim.x * (blockDim.x >> 6);
+  const size_t gw = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   probe_u4 acc = {0u, 0u, 0u, 0u};
-  for (; i + 3 * stride < n16; i += 4 * stride) {
-    const probe_u4 a = __builtin_nontemporal_load(src + i);
-    const probe_u4 b = __builtin_nontemporal_load(src + i + stride);
-    const probe_u4 c = __builtin_nontemporal_load(src + i + 2 * stride);
-    const probe_u4 d = __builtin_nontemporal_load(src + i + 3 * stride);
-    acc ^= a ^ b ^ c ^ d;
+  for (size_t base = gw * 512; base < n16; base += nw * 512) {
+    probe_u4 v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const size_t i = base + 64 * j + lane;
+      v[j] = i < n16 ? __builtin_nontemporal_load(src + i) : probe_u4{0u, 0u, 0u, 0u};
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc ^= v[j];
   }
-  for (; i < n16; i += stride) acc ^= __builtin_nontemporal_load(src + i);
   const uint32_t f = acc.x ^ acc.y ^ acc.z ^ acc.w;
   if (f == 0x9E3779B9u) sink[0] = f;
 }

@@ -26,7 +26,7 @@
 // 2^-11 smaller sum) and the two sums meet once, in the epilogue.
 // v_mfma_f32_16x16x32_f16 runs at the bf16 rate.
 //
-// Data path: a split pass writes zero-padded bf16 planes in a k-blocked
+// Data path: a split pass writes zero-padded 16-bit planes in a k-blocked
 // layout, Ap[S][Kp/32][Mp][32] and Bp[S][Kp/32][Np][32] (B transposed): the
 // 32-wide k-step of a 256-row panel is then 16 KB of contiguous memory, read
 // with whole-line 16-byte loads and no bounds checks.  Block tile 256 x 128 x
@@ -37,6 +37,8 @@
 // reads), and an XCD-aware block order (blocks that share A/B panels run on
 // the same L2).  The plain FP32 kernel keeps K-contiguous FP32 copies and
 // 128 x 128 tiles.
+#include <algorithm>
+
 #include "hml_internal.h"
 #include "hml_dense.h"
 
@@ -64,29 +66,19 @@ __device__ __forceinline__ float bf16_to_float(uint32_t b) {
   return __uint_as_float(b << 16);
 }
 
-// ---- split / pack pass -------------------------------------------------------
-// dst plane s (bf16, or FP32 when S == 0) of a rows x K operand X(r,k) =
-// src[r*sR + k*sK], padded to rowsP x Kp.  One thread per 8 consecutive k of
-// one row.  FP32 copies: the thread index runs fastest along whichever of
-// r / k is contiguous in src, so the reads coalesce.  bf16 planes
-// (k-blocked): thread idx writes the 8 elements at idx * 8 of each plane
-// (chunk q = idx % 4 of row (idx / 4) % rowsP in k-block (idx / 4) / rowsP),
-// so the writes are contiguous and the reads cover 128 bytes of a row (K
-// contiguous) or 4 interleaved runs along the rows (rows contiguous).
-template <int S>
-__global__ void k_split(const float *__restrict__ src, int64_t sR, int64_t sK,
-                        int64_t rows, int64_t K, int64_t rowsP, int64_t Kp,
-                        void *__restrict__ dst, int kfast) {
+// ---- FP32 pack pass -----------------------------------------------------------
+// K-contiguous FP32 copy [rowsP][Kp] of a rows x K operand X(r,k) =
+// src[r*sR + k*sK], zero-padded, for the plain FP32 kernel.  One thread per
+// 8 consecutive k of one row; the thread index runs fastest along whichever
+// of r / k is contiguous in src, so the reads coalesce.
+__global__ void k_pack_f32(const float *__restrict__ src, int64_t sR, int64_t sK,
+                           int64_t rows, int64_t K, int64_t rowsP, int64_t Kp,
+                           float *__restrict__ dst, int kfast) {
   const int64_t kc_n = Kp / 8;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= rowsP * kc_n) return;
   int64_t r, kc;
-  if (S > 0) {
-    const int64_t rq = idx >> 2;
-    const int64_t kt = rq / rowsP;
-    r = rq - kt * rowsP;
-    kc = kt * 4 + (idx & 3);
-  } else if (kfast) {
+  if (kfast) {
     r = idx / kc_n;
     kc = idx - r * kc_n;
   } else {
@@ -99,71 +91,106 @@ __global__ void k_split(const float *__restrict__ src, int64_t sR, int64_t sK,
     const int64_t k = kc * 8 + j;
     v[j] = (r < rows && k < K) ? src[r * sR + k * sK] : 0.f;
   }
-  if (S == 0) {
-    float *d = (float *)dst + r * Kp + kc * 8;
-    *reinterpret_cast<f4 *>(d) = f4{v[0], v[1], v[2], v[3]};
-    *reinterpret_cast<f4 *>(d + 4) = f4{v[4], v[5], v[6], v[7]};
-    return;
-  }
-  const int64_t plane = rowsP * Kp;
+  float *d = dst + r * Kp + kc * 8;
+  *reinterpret_cast<f4 *>(d) = f4{v[0], v[1], v[2], v[3]};
+  *reinterpret_cast<f4 *>(d + 4) = f4{v[4], v[5], v[6], v[7]};
+}
+
+// ---- tiled split passes ------------------------------------------------------
+// The bf16 / fp16 split passes and the row-maximum pass walk the operand in
+// tiles of 64 rows x 32 k (one k-block of the plane layout).  A tile is read
+// with the operand's own contiguous dimension across the lanes -- 16-byte
+// loads, 128 contiguous bytes per row when K is contiguous (LM_K), 256 per
+// k when rows are (LM_R), element loads otherwise (LM_ANY) -- into an LDS
+// tile (row stride 33 floats), then thread t takes row t / 4, k-chunk
+// t % 4 (8 consecutive k): the split writes that chunk's 16 bytes of each
+// plane, and the 256 threads of a block write 64 rows x 64 bytes = 4 KB of
+// every plane contiguously.
+constexpr int kLmK = 0, kLmR = 1, kLmAny = 2;
+constexpr int kSplitRows = 64;
+constexpr int kLdsStride = 33;
+
+// X(r0 + i, k0 + j), i < 64, j < 32, into lt[i * 33 + j]; zero outside the
+// operand.  vec: 16-byte loads are aligned (host-checked).
+template <int LM>
+__device__ __forceinline__ void load_split_tile(const float *__restrict__ src, int64_t sR,
+                                                int64_t sK, int64_t rows, int64_t K,
+                                                int64_t r0, int64_t k0, int vec, float *lt) {
+  const int t = threadIdx.x;
+  if constexpr (LM == kLmK) {
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    uint32_t b[8];
+    for (int it = 0; it < 2; ++it) {
+      const int i = (t >> 3) + 32 * it, j = (t & 7) * 4;
+      const int64_t r = r0 + i, k = k0 + j;
+      f4 v = {0.f, 0.f, 0.f, 0.f};
+      if (r < rows) {
+        const float *p = src + r * sR + k;
+        if (vec && k + 3 < K) {
+          v = *reinterpret_cast<const f4 *>(p);
+        } else {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      b[j] = bf16_bits(v[j]);
-      v[j] = v[j] - bf16_to_float(b[j]);  // exact residual
+          for (int e = 0; e < 4; ++e) v[e] = k + e < K ? p[e] : 0.f;
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) lt[i * kLdsStride + j + e] = v[e];
     }
-    u4 w = {b[0] | (b[1] << 16), b[2] | (b[3] << 16), b[4] | (b[5] << 16),
-            b[6] | (b[7] << 16)};
-    *reinterpret_cast<u4 *>((uint16_t *)dst + s * plane + idx * 8) = w;
+  } else if constexpr (LM == kLmR) {
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      const int j = (t >> 4) + 16 * it, i = (t & 15) * 4;
+      const int64_t r = r0 + i, k = k0 + j;
+      f4 v = {0.f, 0.f, 0.f, 0.f};
+      if (k < K) {
+        const float *p = src + k * sK + r;
+        if (vec && r + 3 < rows) {
+          v = *reinterpret_cast<const f4 *>(p);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) v[e] = r + e < rows ? p[e] : 0.f;
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) lt[(i + e) * kLdsStride + j] = v[e];
+    }
+  } else {
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+      const int idx = t + 256 * it, i = idx >> 5, j = idx & 31;
+      const int64_t r = r0 + i, k = k0 + j;
+      lt[i * kLdsStride + j] = (r < rows && k < K) ? src[r * sR + k * sK] : 0.f;
+    }
   }
 }
 
-// ---- row maxima for the scaled fp16 split ---------------------------------
 // rowmax[r] = max_k |X(r,k)| as FP32 bits (non-negative floats order like
 // their bit patterns, so an integer atomicMax is a float max; NaN ranks
-// above Inf).  rowmax must be zero on entry.  K contiguous (sK == 1): each
-// thread takes 8 consecutive k of one row, the lanes of a wave that share a
-// row combine by a segmented max scan, and the last lane of each row
-// segment issues the atomic.  Otherwise lanes take consecutive rows (sR ==
-// 1 then coalesces) and each thread scans 256 k.
-__global__ void k_rowmax(const float *__restrict__ src, int64_t sR, int64_t sK,
-                         int64_t rows, int64_t K, uint32_t *__restrict__ rowmax,
-                         int kfast) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
+// above Inf).  rowmax must be zero on entry.  Block b covers rows
+// 64 (b % nrb) .. +63 and k-blocks kpb (b / nrb) .. +kpb-1 (nrb row
+// blocks); one atomic per row and block.
+template <int LM>
+__global__ __launch_bounds__(256) void k_rowmax(const float *__restrict__ src, int64_t sR,
+                                                int64_t sK, int64_t rows, int64_t K, int vec,
+                                                int kpb, uint32_t *__restrict__ rowmax) {
+  __shared__ float lt[kSplitRows * kLdsStride];
+  const int t = threadIdx.x, i = t >> 2, q = t & 3;
+  const int64_t nrb = (rows + kSplitRows - 1) / kSplitRows;
+  const int64_t r0 = ((int64_t)blockIdx.x % nrb) * kSplitRows;
+  const int64_t kt0 = ((int64_t)blockIdx.x / nrb) * kpb;
+  const int64_t nkt = (K + kSplitKBlock - 1) / kSplitKBlock;
   uint32_t m = 0;
-  int64_t r;
-  if (kfast) {
-    const int64_t kc_n = (K + 7) / 8;
-    r = idx / kc_n;
-    const int64_t kc = idx - r * kc_n;
-    if (r < rows) {
-      for (int j = 0; j < 8; ++j) {
-        const int64_t k = kc * 8 + j;
-        if (k < K) m = max(m, __float_as_uint(src[r * sR + k]) & 0x7fffffffu);
-      }
-    }
-    // segmented inclusive max scan over lanes of the same row
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t om = __shfl_up(m, off, 64);
-      const int64_t orr = __shfl_up(r, off, 64);
-      if (lane >= off && orr == r) m = max(m, om);
-    }
-    const int64_t nr = __shfl_down(r, 1, 64);
-    const bool last = lane == 63 || nr != r;
-    if (r < rows && last && m) atomicMax(rowmax + r, m);
-    return;
+  for (int64_t kt = kt0; kt < kt0 + kpb && kt < nkt; ++kt) {
+    load_split_tile<LM>(src, sR, sK, rows, K, r0, kt * kSplitKBlock, vec, lt);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      m = max(m, __float_as_uint(lt[i * kLdsStride + q * 8 + j]) & 0x7fffffffu);
+    __syncthreads();
   }
-  const int64_t nkc = (K + 255) / 256;
-  const int64_t kc = idx / rows;
-  r = idx - kc * rows;
-  if (kc >= nkc) return;
-  const int64_t k1 = min(K, (kc + 1) * 256);
-  for (int64_t k = kc * 256; k < k1; ++k)
-    m = max(m, __float_as_uint(src[r * sR + k * sK]) & 0x7fffffffu);
-  if (m) atomicMax(rowmax + r, m);
+  // the 4 threads of a row are adjacent lanes
+  m = max(m, (uint32_t)__shfl_xor((int)m, 1, 64));
+  m = max(m, (uint32_t)__shfl_xor((int)m, 2, 64));
+  if (q == 0 && m && r0 + i < rows) atomicMax(rowmax + r0 + i, m);
 }
 
 // Exponent e with max * 2^e in [2^14, 2^15) (0 for a zero, Inf or NaN max).
@@ -173,31 +200,56 @@ __device__ __forceinline__ int split_exponent(uint32_t maxbits) {
   return 14 - ilogbf(mx);
 }
 
-// Scaled fp16 hi / lo planes in the k-blocked layout of k_split.
-__global__ void k_split_f16(const float *__restrict__ src, int64_t sR, int64_t sK,
-                            int64_t rows, int64_t K, int64_t rowsP, int64_t Kp,
-                            const uint32_t *__restrict__ rowmax,
-                            void *__restrict__ dst) {
-  const int64_t kc_n = Kp / 8;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= rowsP * kc_n) return;
-  const int64_t rq = idx >> 2;
-  const int64_t kt = rq / rowsP;
-  const int64_t r = rq - kt * rowsP;
-  const int64_t kc = kt * 4 + (idx & 3);
-  const int e = r < rows ? split_exponent(rowmax[r]) : 0;
-  _Float16 hi[8], lo[8];
+// bf16 planes (S = 2: hi, lo; S = 3: hi, mid, lo; each residual exact in
+// FP32, rounding to nearest even) or, F16, scaled fp16 hi / lo planes
+// (row r times 2^e_r from rowmax, then hi = fp16(x), lo = fp16(x - hi)),
+// in the k-blocked layout dst[s][Kp/32][rowsP][32].
+template <int S, bool F16, int LM>
+__global__ __launch_bounds__(256) void k_split_tiled(const float *__restrict__ src, int64_t sR,
+                                                     int64_t sK, int64_t rows, int64_t K,
+                                                     int64_t rowsP, int64_t Kp, int vec,
+                                                     const uint32_t *__restrict__ rowmax,
+                                                     void *__restrict__ dst) {
+  __shared__ float lt[kSplitRows * kLdsStride];
+  const int t = threadIdx.x, i = t >> 2, q = t & 3;
+  // block b: row block b % nrb of k-block b / nrb, so consecutive blocks
+  // write consecutive 4 KB pieces of each plane
+  const int64_t nrb = rowsP / kSplitRows;
+  const int64_t r0 = ((int64_t)blockIdx.x % nrb) * kSplitRows;
+  const int64_t kt = (int64_t)blockIdx.x / nrb;
+  load_split_tile<LM>(src, sR, sK, rows, K, r0, kt * kSplitKBlock, vec, lt);
+  __syncthreads();
+  float v[8];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int64_t k = kc * 8 + j;
-    const float x = (r < rows && k < K) ? ldexpf(src[r * sR + k * sK], e) : 0.f;
-    hi[j] = (_Float16)x;
-    lo[j] = (_Float16)(x - (float)hi[j]);  // the residual is exact in FP32
-  }
+  for (int j = 0; j < 8; ++j) v[j] = lt[i * kLdsStride + q * 8 + j];
   const int64_t plane = rowsP * Kp;
-  _Float16 *d = (_Float16 *)dst + idx * 8;
-  *reinterpret_cast<h8 *>(d) = h8{hi[0], hi[1], hi[2], hi[3], hi[4], hi[5], hi[6], hi[7]};
-  *reinterpret_cast<h8 *>(d + plane) = h8{lo[0], lo[1], lo[2], lo[3], lo[4], lo[5], lo[6], lo[7]};
+  const int64_t off = (kt * rowsP + r0 + i) * kSplitKBlock + q * 8;
+  if constexpr (F16) {
+    const int e = r0 + i < rows ? split_exponent(rowmax[r0 + i]) : 0;
+    _Float16 hi[8], lo[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const float x = ldexpf(v[j], e);
+      hi[j] = (_Float16)x;
+      lo[j] = (_Float16)(x - (float)hi[j]);  // the residual is exact in FP32
+    }
+    _Float16 *d = (_Float16 *)dst + off;
+    *reinterpret_cast<h8 *>(d) = h8{hi[0], hi[1], hi[2], hi[3], hi[4], hi[5], hi[6], hi[7]};
+    *reinterpret_cast<h8 *>(d + plane) = h8{lo[0], lo[1], lo[2], lo[3], lo[4], lo[5], lo[6], lo[7]};
+  } else {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      uint32_t b[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        b[j] = bf16_bits(v[j]);
+        v[j] = v[j] - bf16_to_float(b[j]);  // exact residual
+      }
+      u4 w = {b[0] | (b[1] << 16), b[2] | (b[3] << 16), b[4] | (b[5] << 16),
+              b[6] | (b[7] << 16)};
+      *reinterpret_cast<u4 *>((uint16_t *)dst + s * plane + off) = w;
+    }
+  }
 }
 
 // Block index -> (bm, bn): consecutive launch indices rotate over the 8
@@ -452,41 +504,97 @@ __global__ __launch_bounds__(256) void k_dense_f32(DenseParams p) {
 
 }  // namespace
 
+namespace {
+
+// Load mode of a rows x K operand with strides (sR, sK), and whether its
+// 16-byte loads are aligned.
+int split_load_mode(const float *src, int64_t sR, int64_t sK, int64_t rows, int64_t K,
+                    int *vec) {
+  const bool al = (uintptr_t)src % 16 == 0;
+  if (sK == 1 || K == 1) {
+    *vec = al && (sR % 4 == 0 || rows == 1);
+    return kLmK;
+  }
+  if (sR == 1 || rows == 1) {
+    *vec = al && sK % 4 == 0;
+    return kLmR;
+  }
+  *vec = 0;
+  return kLmAny;
+}
+
+template <int S, bool F16>
+hipError_t launch_split_mode(int lm, dim3 grid, hipStream_t st, const float *src, int64_t sR,
+                             int64_t sK, int64_t rows, int64_t K, int64_t rowsP, int64_t Kp,
+                             int vec, const uint32_t *rowmax, void *dst) {
+  switch (lm) {
+    case kLmK: hipLaunchKernelGGL((k_split_tiled<S, F16, kLmK>), grid, dim3(256), 0, st, src, sR, sK, rows, K, rowsP, Kp, vec, rowmax, dst); break;
+    case kLmR: hipLaunchKernelGGL((k_split_tiled<S, F16, kLmR>), grid, dim3(256), 0, st, src, sR, sK, rows, K, rowsP, Kp, vec, rowmax, dst); break;
+    default: hipLaunchKernelGGL((k_split_tiled<S, F16, kLmAny>), grid, dim3(256), 0, st, src, sR, sK, rows, K, rowsP, Kp, vec, rowmax, dst); break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
 hipError_t launch_split(const float *src, int64_t sR, int64_t sK, int64_t rows,
                         int64_t K, int64_t rowsP, int64_t Kp, int S, void *dst,
                         hipStream_t st) {
-  const int64_t total = rowsP * (Kp / 8);
-  if (total <= 0) return hipSuccess;
-  const int kfast = sK == 1 ? 1 : 0;
-  const dim3 grid((unsigned)((total + 255) / 256));
+  if (rowsP <= 0 || Kp <= 0) return hipSuccess;
+  if (S == 0) {
+    const int64_t total = rowsP * (Kp / 8);
+    const int kfast = sK == 1 ? 1 : 0;
+    const dim3 grid((unsigned)((total + 255) / 256));
+    hipLaunchKernelGGL(k_pack_f32, grid, dim3(256), 0, st, src, sR, sK, rows, K, rowsP, Kp,
+                       (float *)dst, kfast);
+    return hipGetLastError();
+  }
+  const int64_t nblk = (rowsP / kSplitRows) * (Kp / kSplitKBlock);
+  if (rowsP % kSplitRows != 0 || Kp % kSplitKBlock != 0 || nblk > 0x7fffffff)
+    return hipErrorInvalidValue;
+  int vec = 0;
+  const int lm = split_load_mode(src, sR, sK, rows, K, &vec);
+  const dim3 grid((unsigned)nblk);
   switch (S) {
-    case 0: hipLaunchKernelGGL(k_split<0>, grid, dim3(256), 0, st, src, sR, sK, rows, K, rowsP, Kp, dst, kfast); break;
-    case 2: hipLaunchKernelGGL(k_split<2>, grid, dim3(256), 0, st, src, sR, sK, rows, K, rowsP, Kp, dst, kfast); break;
-    case 3: hipLaunchKernelGGL(k_split<3>, grid, dim3(256), 0, st, src, sR, sK, rows, K, rowsP, Kp, dst, kfast); break;
+    case 2: return launch_split_mode<2, false>(lm, grid, st, src, sR, sK, rows, K, rowsP, Kp, vec, nullptr, dst);
+    case 3: return launch_split_mode<3, false>(lm, grid, st, src, sR, sK, rows, K, rowsP, Kp, vec, nullptr, dst);
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 hipError_t launch_rowmax(const float *src, int64_t sR, int64_t sK, int64_t rows,
                          int64_t K, uint32_t *rowmax, hipStream_t st) {
   if (rows <= 0 || K <= 0) return hipSuccess;
-  const int kfast = sK == 1 ? 1 : 0;
-  const int64_t total = kfast ? rows * ((K + 7) / 8) : rows * ((K + 255) / 256);
-  const dim3 grid((unsigned)((total + 255) / 256));
-  hipLaunchKernelGGL(k_rowmax, grid, dim3(256), 0, st, src, sR, sK, rows, K, rowmax, kfast);
+  int vec = 0;
+  const int lm = split_load_mode(src, sR, sK, rows, K, &vec);
+  const int64_t nkt = (K + kSplitKBlock - 1) / kSplitKBlock;
+  const int64_t nrb = (rows + kSplitRows - 1) / kSplitRows;
+  // k-blocks per workgroup: up to 8 (fewer atomics), but enough workgroups
+  // (about 4096) to fill the device when the operand is small
+  const int kpb = (int)std::max<int64_t>(1, std::min<int64_t>(8, nrb * nkt / 4096));
+  const int64_t nblk = nrb * ((nkt + kpb - 1) / kpb);
+  if (nblk > 0x7fffffff) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)nblk);
+  switch (lm) {
+    case kLmK: hipLaunchKernelGGL(k_rowmax<kLmK>, grid, dim3(256), 0, st, src, sR, sK, rows, K, vec, kpb, rowmax); break;
+    case kLmR: hipLaunchKernelGGL(k_rowmax<kLmR>, grid, dim3(256), 0, st, src, sR, sK, rows, K, vec, kpb, rowmax); break;
+    default: hipLaunchKernelGGL(k_rowmax<kLmAny>, grid, dim3(256), 0, st, src, sR, sK, rows, K, vec, kpb, rowmax); break;
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_split_f16(const float *src, int64_t sR, int64_t sK, int64_t rows,
                             int64_t K, int64_t rowsP, int64_t Kp, const uint32_t *rowmax,
                             void *dst, hipStream_t st) {
-  const int64_t total = rowsP * (Kp / 8);
-  if (total <= 0) return hipSuccess;
-  const dim3 grid((unsigned)((total + 255) / 256));
-  hipLaunchKernelGGL(k_split_f16, grid, dim3(256), 0, st, src, sR, sK, rows, K, rowsP, Kp,
-                     rowmax, dst);
-  return hipGetLastError();
+  if (rowsP <= 0 || Kp <= 0) return hipSuccess;
+  const int64_t nblk = (rowsP / kSplitRows) * (Kp / kSplitKBlock);
+  if (rowsP % kSplitRows != 0 || Kp % kSplitKBlock != 0 || nblk > 0x7fffffff)
+    return hipErrorInvalidValue;
+  int vec = 0;
+  const int lm = split_load_mode(src, sR, sK, rows, K, &vec);
+  const dim3 grid((unsigned)nblk);
+  return launch_split_mode<2, true>(lm, grid, st, src, sR, sK, rows, K, rowsP, Kp, vec, rowmax,
+                                    dst);
 }
 
 hipError_t launch_dense(int S, const DenseParams &p, hipStream_t st) {
