@@ -9,7 +9,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libhml.so")
+# HML_LIB names another build of the library (the host-sanitizer build of
+# `make -C hiperml_amd/csrc check-asan`)
+LIB_PATH = os.environ.get("HML_LIB") or os.path.join(_HERE, "libhml.so")
 
 c_float_p = ctypes.POINTER(ctypes.c_float)
 c_double_p = ctypes.POINTER(ctypes.c_double)

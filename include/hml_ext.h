@@ -35,8 +35,21 @@ typedef enum {
   HML_STATUS_PARSE_ERROR         = 9
 } hmlStatus;
 
-typedef enum { HML_ROW_MAJOR = 0, HML_COL_MAJOR = 1 } hmlOrder;
-typedef enum { HML_OP_N = 0, HML_OP_T = 1 } hmlOp;
+/* The _ENUM_MIN_ / _ENUM_MAX_ members give the enums the full 32-bit range,
+ * so a C caller's out-of-range value reaches the argument checks (and is
+ * rejected there) instead of being undefined on the C++ side. */
+typedef enum {
+  HML_ROW_MAJOR = 0,
+  HML_COL_MAJOR = 1,
+  HML_ORDER_ENUM_MIN_ = -0x7fffffff - 1,
+  HML_ORDER_ENUM_MAX_ = 0x7fffffff
+} hmlOrder;
+typedef enum {
+  HML_OP_N = 0,
+  HML_OP_T = 1,
+  HML_OP_ENUM_MIN_ = -0x7fffffff - 1,
+  HML_OP_ENUM_MAX_ = 0x7fffffff
+} hmlOp;
 
 /* flags for the Ex entry points */
 #define HML_FLAG_READ_C_ALWAYS 0x1u /* read C (or y) even when beta == 0, as
