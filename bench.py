@@ -50,6 +50,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time budget of the baseline sample")
+    ap.add_argument("--no-tables", action="store_true",
+                    help="do not load the shipped MI355X autotuned tables (selection is then "
+                         "the learned selector, else the heuristic)")
     ap.add_argument("--quick", action="store_true",
                     help="only the shape sweep: leave out the reference-shape, dense "
                          "split-precision and grouped suites of the JSON line")
@@ -96,6 +99,16 @@ def main():
         return
     stream = hml.Stream()
     S = stream.ptr
+    # Selection for the sweep: the exhaustive autotuner's shipped MI355X
+    # tables (a rule of the call's own shape bucket wins), then the learned
+    # selector, then the heuristic.
+    tables = []
+    if not args.no_tables:
+        for kind in ("sgemm", "sgemv"):
+            path = os.path.join(here, "config", "hml_%s_mi355x.cfg" % kind)
+            if os.path.exists(path):
+                hml.config_load(path, kind)
+                tables.append(os.path.relpath(path, here))
 
     # ---- inputs: two copies of every big operand, alternated per call ----
     bufs = {}
@@ -198,7 +211,12 @@ def main():
         gfl = flops / (ms * 1e-3) / 1e9
         ai = flops / nbytes
         roof_gbs = min(HBM_PEAK_GBS, FP32_PEAK_TFLOPS * 1e3 / ai)
-        shapes.append({"call": label, "ms": round(ms, 5), "GB_s": round(gbs, 1),
+        if kind == "sgemv":
+            vv, src = hml.sgemv_select(shp[0], shp[1], trans=shp[2])
+        else:
+            vv, src = hml.sgemm_select(*shp)
+        shapes.append({"call": label, "variant": list(vv.astuple()), "selected_by": src,
+                       "ms": round(ms, 5), "GB_s": round(gbs, 1),
                        "GFLOP_s": round(gfl, 1),
                        "bound": "hbm" if roof_gbs >= HBM_PEAK_GBS else "fp32",
                        "frac_of_roofline": round(gbs / roof_gbs, 4),
@@ -266,7 +284,9 @@ def main():
         "data": "synthetic uniform[-1,1], fixed seeds, inputs resident in HBM",
         "config": {"workload": "plain FP32 shape sweep: SGEMV Ax/ATx 16384x16384 "
                                "and 1048576x64; SGEMM 8192x8192xN N=1..128 (row-major)",
-                   "calls_per_step": len(calls), "alpha": 1.0, "beta": 0.0},
+                   "calls_per_step": len(calls), "alpha": 1.0, "beta": 0.0,
+                   "selection": ("autotuned tables %s, then the learned selector" % tables
+                                 if tables else "learned selector / heuristic")},
         "gflops": round(step_flops / (ms_per_step * 1e-3) / 1e9, 1),
         "wall_s_timed": round(wall, 4),
         "shapes_timing": "per-call events in %d extra steps after the timed ones" % breakdown_steps,

@@ -359,7 +359,7 @@ bool v1_sgemm_variant(const GemmKey &k, hmlSgemmVariant *v) {
   return true;
 }
 
-bool config_select_sgemm(const GemmKey &k, hmlSgemmVariant *v, int *source) {
+bool config_select_sgemm(const GemmKey &k, hmlSgemmVariant *v, int *source, bool exact_only) {
   std::lock_guard<std::mutex> lk(g_cfg_mutex);
   if (!g_sgemm_v2.empty()) {
     const int ml = ilog2(k.M), nl = ilog2(k.N), kl = ilog2(k.K);
@@ -371,7 +371,7 @@ bool config_select_sgemm(const GemmKey &k, hmlSgemmVariant *v, int *source) {
       const int d = abs(r.mlog - ml) + 2 * abs(r.nlog - nl) + abs(r.klog - kl);
       if (d < bestd) { bestd = d; best = (int)i; }
     }
-    if (best >= 0) {
+    if (best >= 0 && (!exact_only || bestd == 0)) {
       *v = g_sgemm_v2[best].v;
       *source = 1;
       return true;
@@ -386,7 +386,7 @@ bool config_select_sgemm(const GemmKey &k, hmlSgemmVariant *v, int *source) {
   return false;
 }
 
-bool config_select_sgemv(const GemvKey &k, hmlSgemvVariant *v, int *source) {
+bool config_select_sgemv(const GemvKey &k, hmlSgemvVariant *v, int *source, bool exact_only) {
   std::lock_guard<std::mutex> lk(g_cfg_mutex);
   if (!g_sgemv_v2.empty()) {
     const int ol = ilog2(k.outputs), rl = ilog2(k.reduced);
@@ -397,7 +397,7 @@ bool config_select_sgemv(const GemvKey &k, hmlSgemvVariant *v, int *source) {
       const int d = abs(r.olog - ol) + abs(r.rlog - rl);
       if (d < bestd) { bestd = d; best = (int)i; }
     }
-    if (best >= 0) {
+    if (best >= 0 && (!exact_only || bestd == 0)) {
       *v = g_sgemv_v2[best].v;
       *source = 1;
       return true;

@@ -305,6 +305,10 @@ bool sgemm_variant_runnable(const GemmKey &k, const hmlSgemmVariant &v) {
 }
 
 void select_sgemm(const GemmKey &k, hmlSgemmVariant *v, int *source) {
+  // A table rule for this exact shape bucket is the exhaustive autotuner's
+  // own measurement: it outranks the learned selector, which generalises
+  // from other shapes; a more distant rule ranks below it.
+  if (config_select_sgemm(k, v, source, true) && sgemm_variant_runnable(k, *v)) return;
   if (learned_select_sgemm(k, v) && sgemm_variant_runnable(k, *v)) {
     *source = 2;
     return;

@@ -82,44 +82,53 @@ void heuristic_sgemv(const GemvKey &k, hmlSgemvVariant *v) {
     v->lpr = lpr_for(ceil_div(k.reduced, loads));
     if (k.reduced >= 512 && k.reduced < 2048) v->lpr = 32;
     v->wpb = 2;
-    const int G = 64 / v->lpr;
     v->rb = v->lpr >= 32 ? (k.outputs >= 8192 ? 2 : 1)
                          : (v->lpr <= 8 && k.outputs >= ((int64_t)1 << 17) ? 4 : 1);
     // Split the rows into enough pieces for about 64K waves when A is at
-    // least 512 MB, 4K below that (a split adds partials and a reduction
-    // pass, which costs more than it hides on mid-size problems), each
-    // piece at least 4 loads per lane long.  The two waves of a workgroup
-    // add their chunks in LDS.  Measured across 16384^2, 32768^2,
+    // least 512 MB, 4K below that (gemv_split_rule).  The two waves of a
+    // workgroup add their chunks in LDS.  Measured across 16384^2, 32768^2,
     // 65536 x 8192 and 8192 x 65536: this two-row, two-wave geometry is the
     // best or within 3% of it; workgroups of 8 waves covering the whole
     // split were up to 7% slower, single-row waves on a 256K-wave split 3%
     // faster on 16384^2 only and up to 7% slower on the others.
-    const int64_t rows_per_wave = (int64_t)G * v->rb;
+    v->split = gemv_split_rule(k, v->lpr, v->rb);
+  } else {
+    // Column chunks of at most 128 floats (32 lanes), 8-wave workgroups
+    // (their waves' sums meet in LDS); the rows split so that about 256
+    // workgroups (2048 waves) run -- 1024 below 64 MB of A (gemv_split_rule).
+    // Measured against 4-wave workgroups on twice the splits over seven
+    // shapes from 4099 x 2049 to 32768^2: 6% faster to 3% slower (16384^2:
+    // 3% faster).
+    v->lpr = std::min(32, lpr_for(k.outputs));
+    v->rb = 1;
+    v->wpb = 8;
+    v->split = gemv_split_rule(k, v->lpr, v->rb);
+  }
+}
+
+int gemv_split_rule(const GemvKey &k, int lpr, int rb) {
+  if (k.family == HML_SGEMV_FAMILY_DOT) {
+    // DOT: about 64K waves when A is at least 512 MB, 4K below that (a
+    // split adds partials and a reduction pass, which costs more than it
+    // hides on mid-size problems), each piece at least 4 loads per lane.
+    const int64_t rows_per_wave = (int64_t)(64 / lpr) * rb;
     const int64_t waves = ceil_div(k.outputs, rows_per_wave);
     const int64_t target = (double)k.outputs * (double)k.reduced * 4.0 >= 536870912.0 ? 65536 : 4096;
     int64_t split = 1;
     if (waves < target) {
       split = ceil_div(target, waves);
-      split = std::min<int64_t>(split, std::max<int64_t>(1, k.reduced / (4 * v->lpr * 4)));
+      split = std::min<int64_t>(split, std::max<int64_t>(1, k.reduced / (4 * lpr * 4)));
     }
-    v->split = (int)std::max<int64_t>(1, std::min<int64_t>(split, 1024));
-  } else {
-    // Column chunks of at most 128 floats (32 lanes), 8-wave workgroups
-    // (their waves' sums meet in LDS); the rows split so that about 256
-    // workgroups (2048 waves) run -- 1024 below 64 MB of A, where each
-    // wave's short row range is latency-bound (4099 x 2049: 18.5 -> 12 us)
-    // -- each split at least 64 rows long, at most 512 splits.  Measured
-    // against 4-wave workgroups on twice the splits over seven shapes from
-    // 4099 x 2049 to 32768^2: 6% faster to 3% slower (16384^2: 3% faster).
-    v->lpr = std::min(32, lpr_for(k.outputs));
-    v->rb = 1;
-    v->wpb = 8;
-    const int64_t ncb = ceil_div(k.outputs, 4 * v->lpr);
-    const int64_t target_wg = (double)k.outputs * (double)k.reduced * 4.0 < 67108864.0 ? 1024 : 256;
-    int64_t split = ceil_div(target_wg, ncb);
-    split = std::min<int64_t>(split, std::max<int64_t>(1, k.reduced / 64));
-    v->split = (int)std::max<int64_t>(1, std::min<int64_t>(split, 512));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(split, 1024));
   }
+  // AXPY: about 256 workgroups -- 1024 below 64 MB of A, where each wave's
+  // short row range is latency-bound (4099 x 2049: 18.5 -> 12 us) -- each
+  // split at least 64 rows long, at most 512 splits.
+  const int64_t ncb = ceil_div(k.outputs, 4 * lpr);
+  const int64_t target_wg = (double)k.outputs * (double)k.reduced * 4.0 < 67108864.0 ? 1024 : 256;
+  int64_t split = ceil_div(target_wg, ncb);
+  split = std::min<int64_t>(split, std::max<int64_t>(1, k.reduced / 64));
+  return (int)std::max<int64_t>(1, std::min<int64_t>(split, 512));
 }
 
 void basic_sgemv(const GemvKey &k, hmlSgemvVariant *v) {
@@ -159,6 +168,13 @@ bool sgemv_variant_runnable(const GemvKey &k, const hmlSgemvVariant &v) {
 }
 
 void select_sgemv(const GemvKey &k, hmlSgemvVariant *v, int *source) {
+  // same order as select_sgemm: exact table bucket, learned selector,
+  // nearest table rule, heuristic
+  if (config_select_sgemv(k, v, source, true) && sgemv_variant_runnable(k, *v)) return;
+  if (learned_select_sgemv(k, v) && sgemv_variant_runnable(k, *v)) {
+    *source = 2;
+    return;
+  }
   if (config_select_sgemv(k, v, source) && sgemv_variant_runnable(k, *v)) return;
   heuristic_sgemv(k, v);
   *source = 0;

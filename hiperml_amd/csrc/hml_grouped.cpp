@@ -21,6 +21,9 @@ struct hmlGroupedPlan_t {
   int64_t cls_off[hml::kGroupClasses] = {};   // tile list of each class
   int64_t cls_n[hml::kGroupClasses] = {};
   int cls_blocks[hml::kGroupClasses] = {};
+  // per class: workgroups take blocks of 4 row tiles that share B through
+  // LDS (true) or each wave takes one row tile with its own copy of B
+  bool cls_shared[hml::kGroupClasses] = {};
   void *dev = nullptr;  // one allocation: problems | tiles
   hml::GroupProblem *probs = nullptr;
   hml::GroupTile *tiles = nullptr;
@@ -38,8 +41,8 @@ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 hmlStatus record(hmlGroupedPlan_t *p, hipStream_t st) {
   for (int c = 0; c < kGroupClasses; ++c) {
     if (!p->cls_n[c]) continue;
-    hipError_t e = launch_grouped(c, p->probs, p->tiles + p->cls_off[c], p->cls_n[c],
-                                  p->cls_blocks[c], st);
+    hipError_t e = launch_grouped(c, p->cls_shared[c], p->probs, p->tiles + p->cls_off[c],
+                                  p->cls_n[c], p->cls_blocks[c], st);
     if (e != hipSuccess) return from_hip(e, "grouped kernel");
   }
   return HML_STATUS_SUCCESS;
@@ -65,6 +68,17 @@ hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan_t **plan, int32_t count,
   std::vector<GroupProblem> probs(count);
   std::vector<int64_t> tile_cost[kGroupClasses];
   std::vector<GroupTile> tiles[kGroupClasses];
+  // Kernel of each class.  Shared B measured faster for N <= 32 (0.93 vs
+  // 0.97 ms and 1.55 vs 1.62 ms on the 512-problem suite's classes) and
+  // slower above (3.28 vs 2.96 ms: its 1 x 4 tiles hold half the rows of
+  // the per-wave 2 x 4 kernel, which fits in registers only without the
+  // LDS staging).  HML_GROUPED_KERNEL=wave / =shared forces one kernel
+  // for every class.
+  bool cls_shared[kGroupClasses] = {true, true, false};
+  if (const char *kenv = getenv("HML_GROUPED_KERNEL")) {
+    if (!strcmp(kenv, "wave") || !strcmp(kenv, "shared"))
+      for (int c = 0; c < kGroupClasses; ++c) cls_shared[c] = kenv[0] == 's';
+  }
   for (int32_t i = 0; i < count; ++i) {
     if (M[i] <= 0 || N[i] <= 0 || K[i] <= 0)
       return fail(HML_STATUS_INVALID_VALUE, "grouped: problem %d has a zero extent", i);
@@ -77,16 +91,17 @@ hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan_t **plan, int32_t count,
     const int cls = N[i] <= 16 ? 0 : (N[i] <= 32 ? 1 : 2);
     int rt, ct;
     grouped_class_shape(cls, &rt, &ct);
+    if (cls_shared[cls]) rt = grouped_shared_rt(cls);
     // the tile body addresses A and B with 31-bit buffer offsets
     if (((16.0 * rt - 1) * lda[i] + K[i] + 4) * 4.0 >= 2147483647.0 ||
-        ((K[i] + 16.0) * ldb[i] + 16.0 * ct) * 4.0 >= 2147483647.0)
+        ((K[i] + 64.0) * ldb[i] + 16.0 * ct) * 4.0 >= 2147483647.0)
       return fail(HML_STATUS_UNSUPPORTED, "grouped: problem %d strides too large", i);
     GroupProblem &p = probs[i];
     p.A = A[i]; p.lda = lda[i]; p.B = B[i]; p.ldb = ldb[i]; p.C = C[i]; p.ldc = ldc[i];
     p.M = M[i]; p.N = N[i]; p.K = K[i];
     p.read_c = beta != 0.0f ? 1 : 0;
     p.alpha = alpha; p.beta = beta;
-    const int64_t rows = 16 * rt;
+    const int64_t rows = 16 * rt * (cls_shared[cls] ? 4 : 1);  // rows of one work unit
     const int64_t nrb = ceil_div(M[i], rows);
     const int64_t passes = ceil_div(ceil_div(N[i], 16), ct);
     // cost of one tile: A bytes per column pass plus C bytes, in 4-byte words
@@ -98,6 +113,7 @@ hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan_t **plan, int32_t count,
   }
   hmlGroupedPlan_t *pl = new hmlGroupedPlan_t();
   pl->count = count;
+  for (int c = 0; c < kGroupClasses; ++c) pl->cls_shared[c] = cls_shared[c];
   std::vector<GroupTile> sorted;
   for (int c = 0; c < kGroupClasses; ++c) {
     // longest-first order (stable: equal costs keep problem order)
@@ -110,9 +126,10 @@ hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan_t **plan, int32_t count,
     for (int64_t t : order) sorted.push_back(tiles[c][t]);
     // persistent grid: the workgroups the device holds at once (4 waves
     // each), never more than needed to give every wave a tile
-    int resident = grouped_class_blocks(c);
+    int resident = grouped_class_blocks(c, cls_shared[c]);
     if (resident <= 0) resident = 512;
-    pl->cls_blocks[c] = (int)std::max<int64_t>(1, std::min<int64_t>(resident, ceil_div(pl->cls_n[c], 4)));
+    pl->cls_blocks[c] = (int)std::max<int64_t>(
+        1, std::min<int64_t>(resident, cls_shared[c] ? pl->cls_n[c] : ceil_div(pl->cls_n[c], 4)));
   }
   pl->ntiles = (int64_t)sorted.size();
   const size_t o_probs = 0;

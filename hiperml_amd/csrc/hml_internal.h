@@ -197,10 +197,14 @@ struct GroupTile {
 // 16), 1: N <= 32 (2 x 2), 2: larger N (1 x 4, several column passes).
 constexpr int kGroupClasses = 3;
 void grouped_class_shape(int cls, int *rt, int *ct);
-// Workgroups of the class kernel the device holds at once (0 if unknown).
-int grouped_class_blocks(int cls);
-hipError_t launch_grouped(int cls, const GroupProblem *probs, const GroupTile *tiles,
-                          int64_t ntiles, int nblocks, hipStream_t st);
+// Workgroups of the class kernel the device holds at once (0 if unknown);
+// shared_b: the kernel whose workgroups take blocks of 4 row tiles and
+// share B through LDS.
+int grouped_class_blocks(int cls, bool shared_b);
+// row tiles (of 16 rows) per wave of the shared-B kernel of a class
+int grouped_shared_rt(int cls);
+hipError_t launch_grouped(int cls, bool shared_b, const GroupProblem *probs,
+                          const GroupTile *tiles, int64_t ntiles, int nblocks, hipStream_t st);
 // k-means assignment (hml_k_kmeans.hip).  Screened form: kKmWaves waves
 // per workgroup, each on its own 16-row tile; NI = dims / 16 rounded up, NT
 // centroid tiles of 16 per screening pass.  Direct form (up to 8 dims, or

@@ -765,42 +765,43 @@ constexpr size_t dotb_lds(int rt, int ct) {
 
 __device__ __forceinline__ int dotb_off(int n, int kq) { return n * kDotStageK + 4 * (kq ^ (n & 15)); }
 
+// One workgroup of 4 waves: wave w computes rows [row0, row0 + 16 RT)
+// (row0 = block row + 16 RT w) x columns [16 ct0, 16 ct0 + 16 CT) over k in
+// [kb, ke) into acc, the block's B stages shared through LDS.  Every wave
+// of the workgroup must call it with the same ct0, kb and ke (it holds
+// workgroup barriers); lds holds dotb_lds(RT, CT) bytes.  On return the
+// LDS may still be read by other waves: barrier before reusing it.
 template <int RT, int CT, bool BVEC>
-__global__ __launch_bounds__(64 * kDotbWaves) void k_skinny_dotb(DotParams p) {
+__device__ __forceinline__ void dotb_block(const float *__restrict__ A, int64_t sAm,
+                                           const float *__restrict__ B, int64_t sBk,
+                                           int64_t sBn, int64_t M, int64_t N, int64_t row0,
+                                           int ct0, int64_t kb, int64_t ke, float *lds,
+                                           f4 (&acc)[RT][CT]) {
   constexpr int PS = 2;
   constexpr int NQ = 4 * RT;        // A loads per stage per lane
   constexpr int TW = RT * 16 * kDotStageK;  // floats of a wave's A tile
   constexpr int BW = CT * 16 * kDotStageK;  // floats of one B stage
-  extern __shared__ f4 dotb_smem[];
-  float *lds = reinterpret_cast<float *>(dotb_smem);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int r = lane & 15, g = lane >> 4;
   float *T = lds + w * TW;
   float *Bs = lds + kDotbWaves * TW;  // [2][16 CT][64]
-  const int64_t rbk = blockIdx.x / p.kgroups;
-  const int kgi = (int)(blockIdx.x - rbk * p.kgroups);
-  const int64_t row0 = (rbk * kDotbWaves + w) * 16 * RT;
-  const int ct0 = blockIdx.y * CT;
   const int64_t n0 = (int64_t)ct0 * 16;
-  const int64_t kb = (int64_t)kgi * p.kchunk;
-  int64_t ke = kb + p.kchunk;
-  if (ke > p.K) ke = p.K;
   const int64_t nst = ke > kb ? (ke - kb) / kDotStageK : 0;
   const bool tail = kb + nst * kDotStageK < ke;
 
   // A: as in dot_tile_lds (rows past M read zero)
-  const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.A + row0 * p.sAm + kb, kRsrcMax);
-  const __amdgpu_buffer_rsrc_t rnull = make_rsrc(p.A, 0);
+  const __amdgpu_buffer_rsrc_t ra = make_rsrc(A + row0 * sAm + kb, kRsrcMax);
+  const __amdgpu_buffer_rsrc_t rnull = make_rsrc(A, 0);
   uint32_t aoff[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q)
-    aoff[q] = row0 + 4 * q + g < p.M ? (uint32_t)(((4 * q + g) * p.sAm + 4 * r) * 4) : kOOB;
+    aoff[q] = row0 + 4 * q + g < M ? (uint32_t)(((4 * q + g) * sAm + 4 * r) * 4) : kOOB;
   // B: thread tid's CT 16-byte pieces of a stage.  K-contiguous B: piece
   // f = tid + 256 i is column n = f / 16, k-quad f % 16.  N-contiguous B:
   // k = f / (4 CT), columns 4 (f % 4CT) .. +3, stored element by element.
-  const __amdgpu_buffer_rsrc_t rb = make_rsrc(BVEC ? p.B + kb + n0 * p.sBn : p.B + kb * p.sBk + n0, kRsrcMax);
+  const __amdgpu_buffer_rsrc_t rb = make_rsrc(BVEC ? B + kb + n0 * sBn : B + kb * sBk + n0, kRsrcMax);
   uint32_t boff[CT];
   int bn[CT], bk[CT];
   bool bwhole[CT];
@@ -811,16 +812,15 @@ __global__ __launch_bounds__(64 * kDotbWaves) void k_skinny_dotb(DotParams p) {
       bn[i] = f >> 4;
       bk[i] = 4 * (f & 15);
       bwhole[i] = true;
-      boff[i] = n0 + bn[i] < p.N ? (uint32_t)((bn[i] * p.sBn + bk[i]) * 4) : kOOB;
+      boff[i] = n0 + bn[i] < N ? (uint32_t)((bn[i] * sBn + bk[i]) * 4) : kOOB;
     } else {
       bk[i] = f / (4 * CT);
       bn[i] = 4 * (f % (4 * CT));
-      bwhole[i] = n0 + bn[i] + 3 < p.N;
-      boff[i] = n0 + bn[i] < p.N ? (uint32_t)((bk[i] * p.sBk + bn[i]) * 4) : kOOB;
+      bwhole[i] = n0 + bn[i] + 3 < N;
+      boff[i] = n0 + bn[i] < N ? (uint32_t)((bk[i] * sBk + bn[i]) * 4) : kOOB;
     }
   }
 
-  f4 acc[RT][CT];
 #pragma unroll
   for (int t = 0; t < RT; ++t)
 #pragma unroll
@@ -835,7 +835,7 @@ __global__ __launch_bounds__(64 * kDotbWaves) void k_skinny_dotb(DotParams p) {
   };
   auto loadB = [&](int j, int64_t s) {
     const __amdgpu_buffer_rsrc_t rs = s < nst ? rb : rnull;
-    const int32_t so = (int32_t)(s * kDotStageK * (BVEC ? 1 : p.sBk) * 4);
+    const int32_t so = (int32_t)(s * kDotStageK * (BVEC ? 1 : sBk) * 4);
 #pragma unroll
     for (int i = 0; i < CT; ++i) {
       if (bwhole[i]) {
@@ -844,7 +844,7 @@ __global__ __launch_bounds__(64 * kDotbWaves) void k_skinny_dotb(DotParams p) {
         f4 x;
 #pragma unroll
         for (int e = 0; e < 4; ++e)
-          x[e] = buf_load1(rs, n0 + bn[i] + e < p.N ? boff[i] + 4 * e : kOOB, so);
+          x[e] = buf_load1(rs, n0 + bn[i] + e < N ? boff[i] + 4 * e : kOOB, so);
         b[j][i] = x;
       }
     }
@@ -940,8 +940,8 @@ __global__ __launch_bounds__(64 * kDotbWaves) void k_skinny_dotb(DotParams p) {
         const int64_t k = BVEC ? k0 + bk[i] + e : k0 + bk[i];
         const int64_t n = n0 + bn[i] + (BVEC ? 0 : e);
         uint32_t vo = kOOB;
-        if (n < p.N && kb + k < ke)
-          vo = BVEC ? (uint32_t)((bn[i] * p.sBn + k) * 4) : (uint32_t)((k * p.sBk + bn[i] + e) * 4);
+        if (n < N && kb + k < ke)
+          vo = BVEC ? (uint32_t)((bn[i] * sBn + k) * 4) : (uint32_t)((k * sBk + bn[i] + e) * 4);
         x[e] = buf_load1(rb, vo, 0);
       }
       b[0][i] = x;
@@ -950,6 +950,24 @@ __global__ __launch_bounds__(64 * kDotbWaves) void k_skinny_dotb(DotParams p) {
     compute((int)(nst & 1));
   }
 
+}
+
+template <int RT, int CT, bool BVEC>
+__global__ __launch_bounds__(64 * kDotbWaves) void k_skinny_dotb(DotParams p) {
+  extern __shared__ f4 dotb_smem[];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int r = lane & 15, g = lane >> 4;
+  const int64_t rbk = blockIdx.x / p.kgroups;
+  const int kgi = (int)(blockIdx.x - rbk * p.kgroups);
+  const int64_t row0 = (rbk * kDotbWaves + w) * 16 * RT;
+  const int ct0 = blockIdx.y * CT;
+  const int64_t kb = (int64_t)kgi * p.kchunk;
+  int64_t ke = kb + p.kchunk;
+  if (ke > p.K) ke = p.K;
+  f4 acc[RT][CT];
+  dotb_block<RT, CT, BVEC>(p.A, p.sAm, p.B, p.sBk, p.sBn, p.M, p.N, row0, ct0, kb, ke,
+                           reinterpret_cast<float *>(dotb_smem), acc);
 #pragma unroll
   for (int t = 0; t < RT; ++t)
 #pragma unroll
@@ -957,6 +975,54 @@ __global__ __launch_bounds__(64 * kDotbWaves) void k_skinny_dotb(DotParams p) {
 #pragma unroll
       for (int i = 0; i < 4; ++i)
         dot_store(p, row0 + 16 * t + 4 * g + i, (int64_t)(ct0 + c) * 16 + r, kgi, acc[t][c][i]);
+}
+
+
+// Grouped SGEMM with B shared through LDS: the work list holds blocks of 4
+// consecutive row tiles (16 RT rows each) of one problem; a workgroup takes
+// blocks b, b + G, ... (G workgroups launched, a static longest-first
+// schedule as in k_grouped_dot) and runs dotb_block on each, its 4 waves one
+// row tile each, so every 64-k stage of the problem's B is read from L2 once
+// per block instead of once per row tile.  Row-major problems only (B is
+// N-contiguous).
+template <int RT, int CT>
+__global__ __launch_bounds__(64 * kDotbWaves, 2) void k_grouped_dotb(const GroupProblem *probs,
+                                                                  const GroupTile *tiles,
+                                                                  int64_t ntiles) {
+  extern __shared__ f4 gdotb_smem[];
+  float *lds = reinterpret_cast<float *>(gdotb_smem);
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int r = lane & 15, g = lane >> 4;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const GroupTile tile = tiles[t];
+    const GroupProblem &gp = probs[tile.prob];
+    const int64_t row0 = (tile.rb * kDotbWaves + w) * 16 * RT;
+    const int64_t nct = (gp.N + 15) / 16;
+    for (int ct0 = 0; ct0 < nct; ct0 += CT) {
+      f4 acc[RT][CT];
+      dotb_block<RT, CT, false>(gp.A, gp.lda, gp.B, gp.ldb, 1, gp.M, gp.N, row0, ct0, 0, gp.K,
+                                lds, acc);
+#pragma unroll
+      for (int tt = 0; tt < RT; ++tt)
+#pragma unroll
+        for (int c = 0; c < CT; ++c)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int64_t row = row0 + 16 * tt + 4 * g + i;
+            const int64_t col = (int64_t)(ct0 + c) * 16 + r;
+            if (row < gp.M && col < gp.N) {
+              float *cp = gp.C + row * gp.ldc + col;
+              float v = gp.alpha * acc[tt][c][i];
+              if (gp.read_c) v = gp.alpha * acc[tt][c][i] + gp.beta * (*cp);
+              *cp = v;
+            }
+          }
+      // the next block's first stage overwrites the LDS other waves may
+      // still be reading
+      __syncthreads();
+    }
+  }
 }
 
 template <int RT, int CT, bool BV>
@@ -1190,16 +1256,31 @@ hipError_t launch_fill(float *dst, size_t n, float value, hipStream_t st) {
 
 namespace hml {
 
+int grouped_shared_rt(int cls);
+
 namespace {
 const void *grouped_fn(int cls) {
   switch (cls) {
     case 0: return (const void *)k_grouped_dot<2, 1>;
     case 1: return (const void *)k_grouped_dot<2, 2>;
     case 2: return (const void *)k_grouped_dot<2, 4>;
+    case kGroupClasses + 0: return (const void *)k_grouped_dotb<2, 1>;
+    case kGroupClasses + 1: return (const void *)k_grouped_dotb<2, 2>;
+    case kGroupClasses + 2: return (const void *)k_grouped_dotb<1, 4>;
     default: return nullptr;
   }
 }
+size_t grouped_lds(int cls) {
+  static const int kCt[kGroupClasses] = {1, 2, 4};
+  return cls >= kGroupClasses
+             ? dotb_lds(grouped_shared_rt(cls - kGroupClasses), kCt[cls - kGroupClasses])
+             : 0;
+}
 }  // namespace
+
+// Row tiles of the shared-B kernel of a class: the 4-column class runs 1 x 4
+// tiles there (2 x 4 needs more than the 256 registers of 2 waves per SIMD)
+int grouped_shared_rt(int cls) { return cls == 2 ? 1 : 2; }
 
 void grouped_class_shape(int cls, int *rt, int *ct) {
   static const int kRt[kGroupClasses] = {2, 2, 2}, kCt[kGroupClasses] = {1, 2, 4};
@@ -1207,22 +1288,28 @@ void grouped_class_shape(int cls, int *rt, int *ct) {
   *ct = kCt[cls];
 }
 
-int grouped_class_blocks(int cls) {
+int grouped_class_blocks(int cls, bool shared_b) {
   static std::mutex mu;
-  static int cache[kGroupClasses];
+  static int cache[2 * kGroupClasses];
+  const int i = cls + (shared_b ? kGroupClasses : 0);
   std::lock_guard<std::mutex> lk(mu);
-  if (!cache[cls]) cache[cls] = resident_blocks(grouped_fn(cls), 0);
-  return cache[cls];
+  if (!cache[i]) cache[i] = resident_blocks(grouped_fn(i), grouped_lds(i));
+  return cache[i];
 }
 
-hipError_t launch_grouped(int cls, const GroupProblem *probs, const GroupTile *tiles,
-                          int64_t ntiles, int nblocks, hipStream_t st) {
+hipError_t launch_grouped(int cls, bool shared_b, const GroupProblem *probs,
+                          const GroupTile *tiles, int64_t ntiles, int nblocks, hipStream_t st) {
   if (ntiles <= 0) return hipSuccess;
   if (nblocks < 1) nblocks = 1;
-  switch (cls) {
-    case 0: hipLaunchKernelGGL((k_grouped_dot<2, 1>), dim3((unsigned)nblocks), dim3(256), 0, st, probs, tiles, ntiles); break;
-    case 1: hipLaunchKernelGGL((k_grouped_dot<2, 2>), dim3((unsigned)nblocks), dim3(256), 0, st, probs, tiles, ntiles); break;
-    case 2: hipLaunchKernelGGL((k_grouped_dot<2, 4>), dim3((unsigned)nblocks), dim3(256), 0, st, probs, tiles, ntiles); break;
+  const dim3 grid((unsigned)nblocks), blk(256);
+  const size_t lds = grouped_lds(cls + (shared_b ? kGroupClasses : 0));
+  switch (cls + (shared_b ? kGroupClasses : 0)) {
+    case 0: hipLaunchKernelGGL((k_grouped_dot<2, 1>), grid, blk, 0, st, probs, tiles, ntiles); break;
+    case 1: hipLaunchKernelGGL((k_grouped_dot<2, 2>), grid, blk, 0, st, probs, tiles, ntiles); break;
+    case 2: hipLaunchKernelGGL((k_grouped_dot<2, 4>), grid, blk, 0, st, probs, tiles, ntiles); break;
+    case kGroupClasses + 0: hipLaunchKernelGGL((k_grouped_dotb<2, 1>), grid, blk, lds, st, probs, tiles, ntiles); break;
+    case kGroupClasses + 1: hipLaunchKernelGGL((k_grouped_dotb<2, 2>), grid, blk, lds, st, probs, tiles, ntiles); break;
+    case kGroupClasses + 2: hipLaunchKernelGGL((k_grouped_dotb<1, 4>), grid, blk, lds, st, probs, tiles, ntiles); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

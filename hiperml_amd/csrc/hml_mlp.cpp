@@ -250,6 +250,27 @@ bool selector_variant(const GemmKey &k, int cls, hmlSgemmVariant *v) {
     heuristic_sgemm(g, v);
     return v->family == HML_SGEMM_FAMILY_GEMV && sgemm_variant_runnable(k, *v);
   }
+  if (cls >= kSelGemvDotFirst) {
+    // explicit GEMV geometries (N == 1 with unit-stride x and y)
+    if (!k.gemv_ok) return false;
+    const bool dot = cls < kSelGemvAxpyFirst;
+    if (dot != (k.form == kFormDot)) return false;
+    int lpr, rb, wpb;
+    if (dot) {
+      const int c = cls - kSelGemvDotFirst;  // lpr x rb x wpb = 4 x 3 x 2
+      lpr = 8 << (c / 6);
+      rb = 1 << ((c / 2) % 3);
+      wpb = (c & 1) ? 8 : 2;
+    } else {
+      const int c = cls - kSelGemvAxpyFirst;  // lpr x wpb = 4 x 2
+      lpr = 8 << (c / 2);
+      rb = 1;
+      wpb = (c & 1) ? 8 : 4;
+    }
+    const GemvKey gk{dot ? HML_SGEMV_FAMILY_DOT : HML_SGEMV_FAMILY_AXPY, k.M, k.K, k.beta0};
+    *v = hmlSgemmVariant{HML_SGEMM_FAMILY_GEMV, lpr, rb, gemv_split_rule(gk, lpr, rb), wpb};
+    return sgemm_variant_runnable(k, *v);
+  }
   if (cls == 86) {
     if (k.form != kFormDot || !(k.N == 2 || k.N == 4 || k.N == 8)) return false;
     GemvKey gk{HML_SGEMV_FAMILY_DOT, k.M, k.K, k.beta0};
@@ -297,14 +318,14 @@ std::once_flag g_autoload_once;
 
 void selector_autoload() {
   std::call_once(g_autoload_once, [] {
-    // Opt-in: HML_SELECTOR=shipped loads the model shipped beside the
-    // library (../config/hml_selector_mi355x.bin), any other non-empty value
-    // other than off / 0 names a model file; unset keeps the table and the
-    // heuristic.
+    // On by default: unset, empty or "shipped" loads the model shipped
+    // beside the library (../config/hml_selector_mi355x.bin); "off" or "0"
+    // keeps the table and the heuristic; any other value names a model
+    // file.
     const char *env = getenv("HML_SELECTOR");
-    if (!env || !*env || !strcmp(env, "off") || !strcmp(env, "0")) return;
+    if (env && (!strcmp(env, "off") || !strcmp(env, "0"))) return;
     std::string path;
-    if (strcmp(env, "shipped") != 0) {
+    if (env && *env && strcmp(env, "shipped") != 0) {
       path = env;
     } else {
       Dl_info info;
@@ -342,13 +363,14 @@ void selector_autoload() {
   });
 }
 
-bool learned_select_sgemm(const GemmKey &k, hmlSgemmVariant *v) {
+bool learned_select_sgemm(const GemmKey &k, hmlSgemmVariant *v, bool gemv_only) {
   selector_autoload();
   std::shared_ptr<const HostNet> net;
-  char key[160];
-  snprintf(key, sizeof key, "%d:%lld:%lld:%lld:%d:%d:%d:%d:%d:%lld:%lld", k.form,
+  char key[168];
+  snprintf(key, sizeof key, "%d:%lld:%lld:%lld:%d:%d:%d:%d:%d:%lld:%lld:%d", k.form,
            (long long)k.M, (long long)k.N, (long long)k.K, k.beta0, k.transA,
-           k.transB, k.order, (int)k.gemv_ok, (long long)k.cM, (long long)k.cN);
+           k.transB, k.order, (int)k.gemv_ok, (long long)k.cM, (long long)k.cN,
+           (int)gemv_only);
   {
     std::lock_guard<std::mutex> lk(g_sel_mutex);
     if (!g_sel_host) return false;
@@ -369,6 +391,7 @@ bool learned_select_sgemm(const GemmKey &k, hmlSgemmVariant *v) {
   for (int c = 0; c < kSelClasses; ++c) {
     hmlSgemmVariant u;
     if (!selector_variant(k, c, &u)) continue;
+    if (gemv_only && u.family != HML_SGEMM_FAMILY_GEMV) continue;
     if (best < 0 || logits[c] > logits[best]) {
       best = c;
       pick = u;
@@ -381,6 +404,25 @@ bool learned_select_sgemm(const GemmKey &k, hmlSgemmVariant *v) {
     if (g_sel_memo.size() >= kSelMemoCap) g_sel_memo.clear();
     g_sel_memo[key] = pick;
   }
+  return true;
+}
+
+// SGEMV through the selector: the problem as the GEMM it is (N = 1, x as a
+// unit-stride column; row-major A x is the dot form, A^T x the axpy form),
+// choosing among the GEMV kernel classes.
+bool learned_select_sgemv(const GemvKey &gk, hmlSgemvVariant *v) {
+  GemmKey k;
+  k.form = gk.family == HML_SGEMV_FAMILY_DOT ? kFormDot : kFormAxpy;
+  k.M = gk.outputs; k.N = 1; k.K = gk.reduced;
+  k.beta0 = gk.beta0;
+  k.transA = gk.family == HML_SGEMV_FAMILY_DOT ? HML_OP_N : HML_OP_T;
+  k.transB = HML_OP_N;
+  k.order = HML_ROW_MAJOR;
+  k.gemv_ok = true;
+  k.cM = gk.outputs; k.cN = 1;
+  hmlSgemmVariant u;
+  if (!learned_select_sgemm(k, &u, true)) return false;
+  *v = hmlSgemvVariant{gk.family, u.rt, u.ct, u.ksplit, u.wpb};
   return true;
 }
 

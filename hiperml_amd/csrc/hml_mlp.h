@@ -32,9 +32,15 @@ hipError_t mlp_gather_rows(const float *src, const int32_t *idx, float *dst,
 // cover N (DOT: at most 4), ksplit 2^0..2^6, wpb 4 or 8); 84: dense FP32
 // tiles; 85: the GEMV kernels (heuristic parameters); 86: the multi-vector
 // GEMV kernels for N = 2, 4, 8 with A contiguous along K (the SGEMV
-// heuristic's parameters for the same rows).  The family of a skinny class
-// follows the problem's form (dot or axpy).
+// heuristic's parameters for the same rows); 87..110: GEMV DOT geometries
+// (lanes per row 8/16/32/64 x rows per lane group 1/2/4 x 2 or 8 waves per
+// workgroup); 111..118: GEMV AXPY geometries (lanes per column chunk
+// 8/16/32/64 x 4 or 8 waves); a GEMV class's row split follows the SGEMV
+// split rule for its geometry.  The family of a skinny class follows the
+// problem's form (dot or axpy).
 constexpr int kSelFeatures = 6;
-constexpr int kSelClasses = 87;
+constexpr int kSelGemvDotFirst = 87;
+constexpr int kSelGemvAxpyFirst = 111;
+constexpr int kSelClasses = 119;
 
 }  // namespace hml

@@ -69,6 +69,9 @@ hmlStatus run_dense(int prec, const float *A, int64_t sAm, int64_t sAk,
                     float alpha, float beta, int read_c, void *ws,
                     size_t ws_bytes, hipStream_t st);
 void heuristic_sgemv(const GemvKey &k, hmlSgemvVariant *v);
+// Row split of a GEMV geometry (lanes per row / column chunk, rows per
+// lane group) for this key: the heuristic's rule for any geometry.
+int gemv_split_rule(const GemvKey &k, int lpr, int rb);
 void basic_sgemm(const GemmKey &k, hmlSgemmVariant *v);
 void basic_sgemv(const GemvKey &k, hmlSgemvVariant *v);
 
@@ -79,15 +82,23 @@ bool sgemm_variant_runnable(const GemmKey &k, const hmlSgemmVariant &v);
 bool sgemv_variant_runnable(const GemvKey &k, const hmlSgemvVariant &v);
 
 // Config-store / learned selection; return false when they have no answer.
-bool config_select_sgemm(const GemmKey &k, hmlSgemmVariant *v, int *source);
-bool config_select_sgemv(const GemvKey &k, hmlSgemvVariant *v, int *source);
+// exact_only: a v2 table answers only for a rule of the key's own shape
+// bucket (a loaded v1 table always answers, as in the reference).
+bool config_select_sgemm(const GemmKey &k, hmlSgemmVariant *v, int *source,
+                         bool exact_only = false);
+bool config_select_sgemv(const GemvKey &k, hmlSgemvVariant *v, int *source,
+                         bool exact_only = false);
 
-// Learned MLP selection (when a trained net is installed).
-bool learned_select_sgemm(const GemmKey &k, hmlSgemmVariant *v);
+// Learned MLP selection (when a trained net is installed).  gemv_only:
+// choose among the GEMV kernel classes only (SGEMV entry points).
+bool learned_select_sgemm(const GemmKey &k, hmlSgemmVariant *v, bool gemv_only = false);
+bool learned_select_sgemv(const GemvKey &k, hmlSgemvVariant *v);
 // Selector class -> variant for a key (false: the class cannot run it).
 bool selector_variant(const GemmKey &k, int cls, hmlSgemmVariant *v);
 
-// Selection entry used by the dispatch (learned -> config -> heuristic).
+// Selection entry used by the dispatch: a table rule of the problem's own
+// shape bucket (the exhaustive autotuner's answer), else the learned
+// selector, else the nearest table rule, else the heuristic.
 void select_sgemm(const GemmKey &k, hmlSgemmVariant *v, int *source);
 void select_sgemv(const GemvKey &k, hmlSgemvVariant *v, int *source);
 

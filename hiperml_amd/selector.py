@@ -26,7 +26,7 @@ import numpy as np
 
 from . import (GemmShape, Mlp, SgemmVariant, check, config_clear, lib,
                sgemm_select, selector_class_of, selector_features,
-               selector_mask)
+               selector_mask, selector_variant)
 
 TAU = 0.05
 
@@ -86,16 +86,18 @@ def autotune(shapes, db_path, trials=3, flush=True, seed=1, progress=None):
 
 def load_db(path):
     """{(M,N,K,transA,transB,beta0): {variant tuple: median ms}} (valid
-    SGEMM rows only)."""
+    SGEMM rows only); `path` may list several databases separated by
+    commas."""
     db = {}
-    with open(path) as f:
-        for r in csv.DictReader(f):
-            if r["kind"] != "0" or r["valid"] != "1":
-                continue
-            key = (int(r["M"]), int(r["N"]), int(r["K"]), int(r["transA"]),
-                   int(r["transB"]), int(r["beta0"]))
-            v = (int(r["family"]), int(r["p1"]), int(r["p2"]), int(r["p3"]), int(r["p4"]))
-            db.setdefault(key, {})[v] = float(r["median_ms"])
+    for one in str(path).split(","):
+        with open(one) as f:
+            for r in csv.DictReader(f):
+                if r["kind"] != "0" or r["valid"] != "1":
+                    continue
+                key = (int(r["M"]), int(r["N"]), int(r["K"]), int(r["transA"]),
+                       int(r["transB"]), int(r["beta0"]))
+                v = (int(r["family"]), int(r["p1"]), int(r["p2"]), int(r["p3"]), int(r["p4"]))
+                db.setdefault(key, {})[v] = float(r["median_ms"])
     return db
 
 
@@ -105,14 +107,23 @@ def shape_kw(key):
 
 
 def class_times(key, times):
-    """Per selector class: measured median ms (inf where not measured)."""
+    """Per selector class: measured median ms (inf where not measured).  A
+    class whose variant for this shape equals a measured variant of another
+    class (two geometries the split rule makes identical, a GEMV class that
+    is the heuristic's own pick) takes that variant's time."""
     M, N, K = key[:3]
+    kw = shape_kw(key)
     C = lib.hmlSelectorClassCount()
     t = np.full(C, np.inf)
     for v, ms in times.items():
-        c = selector_class_of(M, N, K, v, **shape_kw(key))
+        c = selector_class_of(M, N, K, v, **kw)
         if c >= 0:
             t[c] = min(t[c], ms)
+    for c in np.nonzero(selector_mask(M, N, K, **kw) > 0)[0]:
+        if not np.isfinite(t[c]):
+            v = selector_variant(M, N, K, int(c), **kw)
+            if v is not None and v.astuple() in times:
+                t[c] = times[v.astuple()]
     return t
 
 
@@ -241,9 +252,33 @@ def run(out_dir, n_train, n_eval, seed=2024, epochs=300, trials=3, mem_cap=8e9,
     return rep
 
 
+def collect(out_dir, n, seed, name, trials=3, mem_cap=8e9, time_limit=None):
+    """Autotunes `n` random shapes of stream `seed` into out_dir/name.csv
+    (resuming: shapes already in that database cost nothing), stopping
+    early after time_limit seconds."""
+    import os
+    import time
+    os.makedirs(out_dir, exist_ok=True)
+    db = os.path.join(out_dir, name + ".csv")
+    shapes = random_shapes(n, seed, mem_cap)
+    t0 = time.time()
+    done = 0
+    for i in range(0, len(shapes), 25):
+        if time_limit and time.time() - t0 > time_limit:
+            break
+        done += autotune(shapes[i:i + 25], db, trials=trials, seed=seed * 1000003 + i)
+        print("collected %d/%d shapes into %s (%.0f s)" % (min(i + 25, n), n, db,
+                                                         time.time() - t0), flush=True)
+    return done
+
+
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description="train and evaluate the learned kernel selector")
+    ap.add_argument("--collect", metavar="NAME",
+                    help="only autotune --train random shapes of stream --seed into "
+                         "OUT/NAME.csv (resumable), then stop")
+    ap.add_argument("--time-limit", type=float, help="with --collect: stop after this many seconds")
     ap.add_argument("--out", default="selector_out")
     ap.add_argument("--train", type=int, default=500, help="random training shapes")
     ap.add_argument("--eval", type=int, default=2000, help="random held-out shapes")
@@ -261,6 +296,9 @@ def main(argv=None):
     import os
     os.environ["HML_SELECTOR"] = "off"
     check(lib.hmlInit())
+    if a.collect:
+        collect(a.out, a.train, a.seed, a.collect, a.trials, a.mem_cap, a.time_limit)
+        return
     run(a.out, a.train, a.eval, a.seed, a.epochs, a.trials, a.mem_cap, a.train_db, a.eval_db,
         not a.no_named)
 

@@ -32,13 +32,26 @@ def exact(A, B):
     return A.astype(np.int64) @ B.astype(np.int64)
 
 
-def test_grouped_exact_direct(hml):
+@pytest.mark.parametrize("kernel", ["default", "shared", "wave"])
+def test_grouped_exact_direct(hml, kernel, monkeypatch):
+    """Every grouped kernel.  Shared-B workgroups take blocks of 4 row tiles
+    and share B through LDS (128 rows per block for N <= 32, 64 above);
+    per-wave kernels take one 32-row tile per wave.  The default uses the
+    shared-B kernel for N <= 32 and the per-wave kernel above;
+    HML_GROUPED_KERNEL=shared / =wave forces one kind for every class."""
+    if kernel != "default":
+        monkeypatch.setenv("HML_GROUPED_KERNEL", kernel)
     rng = np.random.default_rng(10)
     probs, host = make_problems(hml, rng, 40)
     plan = hml.GroupedPlan(probs, 1.0, 0.0)
     info = plan.info()
-    # every tile class (by N: 16, 32 or 64 columns per pass) has 32-row tiles
-    assert info["tiles"] == sum((p[0] + 31) // 32 for p in probs)
+    if kernel == "wave":
+        unit = lambda n: 32
+    elif kernel == "shared":
+        unit = lambda n: 64 if n > 32 else 128
+    else:
+        unit = lambda n: 32 if n > 32 else 128
+    assert info["tiles"] == sum((p[0] + unit(p[1]) - 1) // unit(p[1]) for p in probs)
     plan.run(graph=False)
     hml.synchronize()
     for (M, N, K, dA, dB, dC), (A, B, C) in zip(probs, host):

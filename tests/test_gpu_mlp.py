@@ -119,10 +119,11 @@ def test_selector_install_and_fallback(hml, tmp_path):
 
 
 def test_shipped_selector_autoload(hml, tmp_path):
-    """With HML_SELECTOR naming a model file, or `shipped` (the model under
-    config/), the first selection in a fresh process installs it (source 2 =
-    learned); HML_SELECTOR=off or unset keeps the heuristic.  Runs in child
-    processes so this process keeps its state."""
+    """With HML_SELECTOR naming a model file, `shipped`, or unset (the model
+    under config/, when it matches the library's class space), the first
+    selection in a fresh process installs it (source 2 = learned);
+    HML_SELECTOR=off or 0 keeps the heuristic.  Runs in child processes so
+    this process keeps its state."""
     import subprocess
     import sys
     import os
@@ -133,8 +134,13 @@ def test_shipped_selector_autoload(hml, tmp_path):
             "h.check(h.lib.hmlInit()); v, s = h.sgemm_select(1 << 16, 16, 4096); "
             "print('SRC', s)" % os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    shipped = os.path.exists(os.path.join(root, "config", "hml_selector_mi355x.bin"))
-    cases = [(path, 2), ("off", 0), (None, 0)] + ([("shipped", 2)] if shipped else [])
+    shipped_path = os.path.join(root, "config", "hml_selector_mi355x.bin")
+    shipped = False
+    if os.path.exists(shipped_path):
+        hdr = np.fromfile(shipped_path, np.int32, 5)
+        shipped = int(hdr[4]) == hml.lib.hmlSelectorClassCount()
+    cases = [(path, 2), ("off", 0), ("0", 0), (None, 2 if shipped else 0),
+             ("shipped", 2 if shipped else 0)]
     for setting, want in cases:
         env = dict(os.environ)
         env.pop("HML_SELECTOR", None)
