@@ -639,13 +639,19 @@ def bench_host(hml, reps=3):
             y = np.zeros(M, np.float32)
             fn = lambda: hml.sgemv(y, A, x, 1.0, 0.0)
             nbytes = 4.0 * (M * N + M + N)
-        fn()
-        t0 = time.perf_counter()
-        for _ in range(reps):
+        res = {"call": "%s %s (host pointers)" % (kind, "x".join(map(str, shape)))}
+        # staged: pinned double-buffered chunks (the default); plain: one
+        # hipMemcpy per operand (HML_HOST_STAGING=off)
+        for mode, env in (("staged", "on"), ("plain", "off")):
+            os.environ["HML_HOST_STAGING"] = env
             fn()
-        dt = (time.perf_counter() - t0) / reps
-        out.append({"call": "%s %s (host pointers)" % (kind, "x".join(map(str, shape))),
-                    "ms": round(dt * 1e3, 3), "GB_s": round(nbytes / dt / 1e9, 1)})
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            dt = (time.perf_counter() - t0) / reps
+            res[mode] = {"ms": round(dt * 1e3, 3), "GB_s": round(nbytes / dt / 1e9, 1)}
+        os.environ.pop("HML_HOST_STAGING", None)
+        out.append(res)
     print(json.dumps({"suite": "host-pointer entry points, PCIe copies included", "results": out}))
 
 

@@ -517,9 +517,9 @@ void sgemm_compat(float *C, const float *A, const float *B, int M, int N,
   std::lock_guard<std::mutex> lk(compat_mutex());
   hmlStatus s = compat_buffers(szA * 4, szB * 4, szC * 4, &dA, &dB, &dC);
   if (s != HML_STATUS_SUCCESS) compat_die("alloc A, B, C", s);
-  if ((s = hmlMemcpyHtoD(dA, A, szA * 4)) != HML_STATUS_SUCCESS) compat_die("copy A", s);
-  if ((s = hmlMemcpyHtoD(dB, B, szB * 4)) != HML_STATUS_SUCCESS) compat_die("copy B", s);
-  if ((s = hmlMemcpyHtoD(dC, C, szC * 4)) != HML_STATUS_SUCCESS) compat_die("copy C", s);
+  if ((s = compat_h2d(dA, A, szA * 4)) != HML_STATUS_SUCCESS) compat_die("copy A", s);
+  if ((s = compat_h2d(dB, B, szB * 4)) != HML_STATUS_SUCCESS) compat_die("copy B", s);
+  if ((s = compat_h2d(dC, C, szC * 4)) != HML_STATUS_SUCCESS) compat_die("copy C", s);
   hmlSgemmVariant v;
   GemmCanon g = canonicalize(HML_ROW_MAJOR, HML_OP_N, HML_OP_N, M, N, K, dA,
                              std::max(K, 1), dB, std::max(N, 1), dC, std::max(N, 1));
@@ -541,7 +541,7 @@ void sgemm_compat(float *C, const float *A, const float *B, int M, int N,
     fprintf(stderr, "Run kernel: %s\n", hmlLastError());
     exit(1);
   }
-  if ((s = hmlMemcpyDtoH(C, dC, szC * 4)) != HML_STATUS_SUCCESS) {
+  if ((s = compat_d2h(C, dC, szC * 4)) != HML_STATUS_SUCCESS) {
     fprintf(stderr, "Copy C off of device: %s\n", hmlLastError());
     exit(1);
   }

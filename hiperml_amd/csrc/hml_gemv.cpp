@@ -265,9 +265,9 @@ void sgemv_compat(float *y, const float *A, const float *x, int M, int N,
   std::lock_guard<std::mutex> lk(compat_mutex());
   hmlStatus s = compat_buffers(szA * 4, (size_t)N * 4, (size_t)M * 4, &dA, &dx, &dy);
   if (s != HML_STATUS_SUCCESS) compat_die("alloc A, x, y", s);
-  if ((s = hmlMemcpyHtoD(dA, A, szA * 4)) != HML_STATUS_SUCCESS) compat_die("copy A", s);
-  if ((s = hmlMemcpyHtoD(dx, x, (size_t)N * 4)) != HML_STATUS_SUCCESS) compat_die("copy x", s);
-  if ((s = hmlMemcpyHtoD(dy, y, (size_t)M * 4)) != HML_STATUS_SUCCESS) compat_die("copy y", s);
+  if ((s = compat_h2d(dA, A, szA * 4)) != HML_STATUS_SUCCESS) compat_die("copy A", s);
+  if ((s = compat_h2d(dx, x, (size_t)N * 4)) != HML_STATUS_SUCCESS) compat_die("copy x", s);
+  if ((s = compat_h2d(dy, y, (size_t)M * 4)) != HML_STATUS_SUCCESS) compat_die("copy y", s);
   GemvKey key{HML_SGEMV_FAMILY_DOT, M, N, 0};
   hmlSgemvVariant v;
   int src = 0;
@@ -282,7 +282,7 @@ void sgemv_compat(float *y, const float *A, const float *x, int M, int N,
     fprintf(stderr, "Run kernel: %s\n", hmlLastError());
     exit(1);
   }
-  if ((s = hmlMemcpyDtoH(y, dy, (size_t)M * 4)) != HML_STATUS_SUCCESS) {
+  if ((s = compat_d2h(y, dy, (size_t)M * 4)) != HML_STATUS_SUCCESS) {
     fprintf(stderr, "Copy y off of device: %s\n", hmlLastError());
     exit(1);
   }

@@ -128,9 +128,6 @@ __device__ __forceinline__ void dot_store(const DotParams &p, int64_t row,
 
 // One wave: rows [row0, row0 + 16 RT) x columns [ct0*16, ct0*16 + 16 CT)
 // over k in [kb, ke) (kb a multiple of the 64-k stage), accumulated into
-// acc; T is the wave's private LDS tile (16 RT x 64 floats).
-// One wave: rows [row0, row0 + 16 RT) x columns [ct0*16, ct0*16 + 16 CT)
-// over k in [kb, ke) (kb a multiple of the 64-k stage), accumulated into
 // acc.  T is the wave's private LDS: one A tile (16 RT x 64 floats) when the
 // stages go through registers, PS tiles when DMA.
 //

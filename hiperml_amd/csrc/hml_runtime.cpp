@@ -11,10 +11,14 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
+#include <algorithm>
+#include <condition_variable>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "hml_internal.h"
@@ -172,6 +176,172 @@ hmlStatus compat_buffers(size_t a_bytes, size_t b_bytes, size_t c_bytes,
   *b = (float *)(base + al(a_bytes));
   *c = (float *)(base + al(a_bytes) + al(b_bytes));
   return HML_STATUS_SUCCESS;
+}
+
+// Staged host <-> device copies.  The reference copies its host operands
+// with one cudaMemcpy each (src/hml_utils.cu:147-230).  A pageable array
+// cannot be read by the DMA engines, and registering an existing host
+// allocation is not used here, so the bytes pass through pinned memory
+// anyway; doing that staging here lets several host threads copy into one
+// pinned chunk while the DMA engine moves the previous one.
+namespace {
+constexpr size_t kStageChunk = (size_t)32 << 20;  // bytes per pinned buffer
+constexpr size_t kStageMin = (size_t)4 << 20;     // smaller copies: hipMemcpy
+
+// Persistent helper threads that split one memcpy between themselves and
+// the calling thread.  Created on first use and never destroyed (they wait
+// on a condition variable until the process ends).
+class CopyPool {
+ public:
+  explicit CopyPool(int helpers) {
+    for (int i = 0; i < helpers; ++i) th_.emplace_back([this, i] { run(i + 1); });
+  }
+  void copy(void *dst, const void *src, size_t n) {
+    {
+      std::lock_guard<std::mutex> l(m_);
+      dst_ = (char *)dst;
+      src_ = (const char *)src;
+      n_ = n;
+      pending_ = (int)th_.size();
+      ++gen_;
+    }
+    cv_.notify_all();
+    slice(0);
+    std::unique_lock<std::mutex> l(m_);
+    done_.wait(l, [&] { return pending_ == 0; });
+  }
+
+ private:
+  void slice(int i) const {
+    const size_t parts = th_.size() + 1;
+    const size_t per = ((n_ + parts - 1) / parts + 255) & ~(size_t)255;
+    const size_t b = std::min(n_, per * (size_t)i), e = std::min(n_, b + per);
+    if (e > b) memcpy(dst_ + b, src_ + b, e - b);
+  }
+  void run(int i) {
+    uint64_t seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> l(m_);
+        cv_.wait(l, [&] { return gen_ != seen; });
+        seen = gen_;
+      }
+      slice(i);
+      std::lock_guard<std::mutex> l(m_);
+      if (--pending_ == 0) done_.notify_one();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex m_;
+  std::condition_variable cv_, done_;
+  uint64_t gen_ = 0;
+  int pending_ = 0;
+  char *dst_ = nullptr;
+  const char *src_ = nullptr;
+  size_t n_ = 0;
+};
+
+CopyPool &copy_pool() {
+  static CopyPool *p = [] {
+    // host threads per staged chunk (the caller included); a handful
+    // saturate one PCIe link
+    int t = 6;
+    if (const char *s = getenv("HML_COPY_THREADS")) t = atoi(s);
+    const int hw = (int)std::thread::hardware_concurrency();
+    if (hw > 0) t = std::min(t, hw);
+    return new CopyPool(std::max(1, std::min(t, 32)) - 1);
+  }();
+  return *p;
+}
+
+struct HostStage {
+  void *pinned[2] = {nullptr, nullptr};
+  hipStream_t st = nullptr;
+  hipEvent_t done[2] = {nullptr, nullptr};
+};
+std::vector<HostStage> g_stage;  // per device, guarded by g_compat_mutex
+
+bool staging_off() {
+  const char *s = getenv("HML_HOST_STAGING");
+  return s && (!strcmp(s, "off") || !strcmp(s, "0"));
+}
+
+hmlStatus host_stage(HostStage **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return from_hip(e, "hipGetDevice");
+  if ((int)g_stage.size() <= dev) g_stage.resize(dev + 1);
+  HostStage &h = g_stage[dev];
+  if (!h.st) {
+    hipStream_t st = nullptr;
+    if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess)
+      return from_hip(e, "staging stream");
+    HostStage n;
+    n.st = st;
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+      e = hipHostMalloc(&n.pinned[i], kStageChunk, hipHostMallocDefault);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&n.done[i], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+      for (int i = 0; i < 2; ++i) {
+        if (n.pinned[i]) (void)hipHostFree(n.pinned[i]);
+        if (n.done[i]) (void)hipEventDestroy(n.done[i]);
+      }
+      (void)hipStreamDestroy(st);
+      return fail(HML_STATUS_ALLOC_FAILED, "pinned staging buffers: %s", hipGetErrorString(e));
+    }
+    h = n;
+  }
+  *out = &h;
+  return HML_STATUS_SUCCESS;
+}
+}  // namespace
+
+hmlStatus compat_h2d(void *dst, const void *src, size_t bytes) {
+  if (bytes < kStageMin || staging_off())
+    return from_hip(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+  HostStage *h = nullptr;
+  hmlStatus s = host_stage(&h);
+  if (s != HML_STATUS_SUCCESS) return s;
+  hipError_t e = hipSuccess;
+  size_t i = 0;
+  for (size_t off = 0; off < bytes && e == hipSuccess; off += kStageChunk, ++i) {
+    const int b = (int)(i & 1);
+    const size_t n = std::min(kStageChunk, bytes - off);
+    // the DMA read of this buffer's previous chunk must be done
+    if (i >= 2 && (e = hipEventSynchronize(h->done[b])) != hipSuccess) break;
+    copy_pool().copy(h->pinned[b], (const char *)src + off, n);
+    e = hipMemcpyAsync((char *)dst + off, h->pinned[b], n, hipMemcpyHostToDevice, h->st);
+    if (e == hipSuccess) e = hipEventRecord(h->done[b], h->st);
+  }
+  const hipError_t e2 = hipStreamSynchronize(h->st);
+  return from_hip(e != hipSuccess ? e : e2, "staged copy H2D");
+}
+
+hmlStatus compat_d2h(void *dst, const void *src, size_t bytes) {
+  if (bytes < kStageMin || staging_off())
+    return from_hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+  HostStage *h = nullptr;
+  hmlStatus s = host_stage(&h);
+  if (s != HML_STATUS_SUCCESS) return s;
+  const size_t nchunk = (bytes + kStageChunk - 1) / kStageChunk;
+  auto issue = [&](size_t i) {
+    const size_t off = i * kStageChunk, n = std::min(kStageChunk, bytes - off);
+    hipError_t e = hipMemcpyAsync(h->pinned[i & 1], (const char *)src + off, n,
+                                  hipMemcpyDeviceToHost, h->st);
+    return e == hipSuccess ? hipEventRecord(h->done[i & 1], h->st) : e;
+  };
+  hipError_t e = issue(0);
+  for (size_t i = 0; i < nchunk && e == hipSuccess; ++i) {
+    // chunk i + 1 lands in the other buffer, whose chunk i - 1 the host
+    // has already copied out
+    if (i + 1 < nchunk && (e = issue(i + 1)) != hipSuccess) break;
+    if ((e = hipEventSynchronize(h->done[i & 1])) != hipSuccess) break;
+    const size_t off = i * kStageChunk;
+    copy_pool().copy((char *)dst + off, h->pinned[i & 1], std::min(kStageChunk, bytes - off));
+  }
+  const hipError_t e2 = hipStreamSynchronize(h->st);
+  return from_hip(e != hipSuccess ? e : e2, "staged copy D2H");
 }
 
 }  // namespace hml

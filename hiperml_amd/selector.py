@@ -21,11 +21,12 @@ The pipeline (all timings and all training run on the GPU through libhml):
 import csv
 import ctypes
 import math
+import os
 
 import numpy as np
 
-from . import (GemmShape, Mlp, SgemmVariant, check, config_clear, lib,
-               sgemm_select, selector_class_of, selector_features,
+from . import (GemmShape, Mlp, SgemmVariant, check, config_clear, config_load, lib,
+               selector_install, sgemm_select, selector_class_of, selector_features,
                selector_mask, selector_variant)
 
 TAU = 0.05
@@ -61,6 +62,10 @@ def random_shapes(n, seed, mem_cap=8e9, lo=16, hi=65536, nlo=1, nhi=512):
         out.append((M, N, K, ta, tb, b0))
     return out
 
+
+# The shipped MI355X SGEMM table (hiperml_amd.tables writes it).
+SHIPPED_TABLE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                             "config", "hml_sgemm_mi355x.cfg")
 
 # Row-major 8192 x N x 8192 problems, beta = 0: the skinny-SGEMM sweep.
 NAMED_SHAPES = [(8192, n, 8192, 0, 0, 1) for n in (1, 2, 4, 8, 16, 32, 64, 128)]
@@ -243,7 +248,8 @@ def run(out_dir, n_train, n_eval, seed=2024, epochs=300, trials=3, mem_cap=8e9,
            "eval_shapes": len(dbe), "epochs": epochs,
            "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
            "fraction_of_autotune_best": summarize(fr),
-           "named_shapes": named_report(net, load_db(named_db), dbt) if named_db else None,
+           "named_shapes": named_report(net, load_db(named_db), dbt, SHIPPED_TABLE)
+           if named_db else None,
            "seconds": round(time.time() - t0, 1)}
     net.save(os.path.join(out_dir, "hml_selector_mi355x.bin"))
     with open(os.path.join(out_dir, "selector_report.json"), "w") as f:
@@ -303,13 +309,31 @@ def main(argv=None):
         not a.no_named)
 
 
-def named_report(net, db_named, db_train):
+def named_report(net, db_named, db_train, shipped_table=None):
     """Per named shape: the fraction of the autotune-best time reached by
-    each selector."""
+    each selector; with `shipped_table` (a v2 .cfg file) also by the
+    library's full selection order with that table loaded and `net`
+    installed ("shipped": an exact-bucket rule, else the network)."""
+    import os
     out = {}
     for key in sorted(db_named):
         fr = evaluate(net, {key: db_named[key]}, db_train)
         out["%dx%dx%d" % key[:3]] = {k: round(float(v[0]), 4) for k, v in fr.items() if len(v)}
+    if shipped_table and os.path.exists(shipped_table) and net is not None:
+        config_clear()
+        config_load(shipped_table)
+        selector_install(net)
+        try:
+            for key in sorted(db_named):
+                M, N, K, ta, tb, b0 = key
+                times = db_named[key]
+                v, src = sgemm_select(M, N, K, transA=bool(ta), transB=bool(tb),
+                                      beta=0.0 if b0 else 1.0)
+                t = times.get(v.astuple(), math.inf)
+                out["%dx%dx%d" % key[:3]]["shipped"] = round(best_time(times) / t, 4)
+        finally:
+            selector_install(None)
+            config_clear()
     return out
 
 

@@ -258,6 +258,25 @@ def test_reference_cfg1_compat(hml):
     assert_within_gamma(Cb, R, bound, K)
 
 
+@pytest.mark.parametrize("M,N,K", [(3000, 24, 5000), (70000, 300, 16)])
+def test_compat_staged_copies(hml, monkeypatch, M, N, K):
+    """hmlSgemm moves host operands of 4 MiB and more through pinned
+    32 MiB chunks (A of 57 MiB: two chunks; C of 80 MiB: three, so both
+    buffers are reused).  Integer-valued inputs: the result equals the
+    exact product bit for bit, with staging on and with
+    HML_HOST_STAGING=off (plain hipMemcpy)."""
+    rng = np.random.default_rng(11)
+    A = int_matrix(rng, (M, K))
+    B = int_matrix(rng, (K, N))
+    C0 = int_matrix(rng, (M, N))
+    R = (A.astype(np.int64) @ B.astype(np.int64) * 2 - C0.astype(np.int64)).astype(np.float32)
+    for mode in ("on", "off"):
+        monkeypatch.setenv("HML_HOST_STAGING", mode)
+        C = C0.copy()
+        hml.sgemm(C, A, B, 2.0, -1.0)
+        np.testing.assert_array_equal(C, R, err_msg="staging " + mode)
+
+
 def test_compat_reads_c_when_beta_zero(hml):
     """The reference kernels always read C (beta*C with beta == 0), so a NaN
     in C propagates (include/hml_sgemm_kernel_template.h:187).  The

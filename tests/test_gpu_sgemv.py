@@ -115,6 +115,23 @@ def test_random_vs_fp64_layouts(hml):
                 assert_within_gamma(y[:, None], R, bound, nx)
 
 
+def test_compat_staged_copies(hml, monkeypatch):
+    """hmlSgemv with a 92 MiB host A (three pinned 32 MiB chunks, both
+    staging buffers reused): integer-valued inputs give the exact result,
+    with staging on and with HML_HOST_STAGING=off."""
+    rng = np.random.default_rng(12)
+    M, N = 6000, 4000
+    A = int_matrix(rng, (M, N))
+    x = int_matrix(rng, (N,))
+    y0 = int_matrix(rng, (M,))
+    R = (A.astype(np.int64) @ x.astype(np.int64) + 3 * y0.astype(np.int64)).astype(np.float32)
+    for mode in ("on", "off"):
+        monkeypatch.setenv("HML_HOST_STAGING", mode)
+        y = y0.copy()
+        hml.sgemv(y, A, x, 1.0, 3.0)
+        np.testing.assert_array_equal(y, R, err_msg="staging " + mode)
+
+
 def test_compat_reads_y_when_beta_zero(hml):
     rng = np.random.default_rng(4)
     A = uniform(rng, (64, 40))
