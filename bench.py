@@ -498,7 +498,9 @@ def grouped_suite_shapes(count=512, seed=0):
 def bench_grouped(hml, count=512, reps=5, seed=0):
     """512 uneven skinny problems (M log-uniform [64,65536], K log-uniform
     [64,4096], N uniform in 1..64, uniform[-1,1]) in one persistent grouped
-    launch (direct and hipGraph) vs 512 sequential hmlSgemmEx calls."""
+    launch (direct and hipGraph) vs 512 sequential hmlSgemmEx calls; the
+    plan's kernel kinds are chosen by hmlSgemmGroupedAutotune before the
+    timed runs (the default kinds' graph time is reported beside them)."""
     import numpy as np
     S = hml.Stream()
     probs, nbytes, flops = [], 0.0, 0.0
@@ -529,6 +531,11 @@ def bench_grouped(hml, count=512, reps=5, seed=0):
         return e0.elapsed_ms(e1) / reps
 
     t_seq = timed(seq)
+    default_kinds = plan.kinds()
+    t_default = timed(lambda: plan.run(graph=True, stream=S.ptr))
+    # tile shapes chosen by the plan-level autotuner (every kernel kind of
+    # every class timed on these problems; C is restored afterwards)
+    tune = plan.autotune(reps=3, stream=S.ptr)
     t_dir = timed(lambda: plan.run(graph=False, stream=S.ptr))
     t_graph = timed(lambda: plan.run(graph=True, stream=S.ptr))
     info = plan.info()
@@ -552,6 +559,10 @@ def bench_grouped(hml, count=512, reps=5, seed=0):
            "grouped_graph_GB_s": round(nbytes / t_graph / 1e6, 1),
            "roofline_floor_ms": round(floor_ms, 4),
            "frac_of_roofline": round(floor_ms / t_graph, 4),
+           "kinds": tune["kinds"],
+           "autotune_class_ms": {str(c): {k: round(v, 4) for k, v in t.items()}
+                                 for c, t in tune["ms"].items()},
+           "default_kinds": default_kinds, "default_kinds_graph_ms": round(t_default, 4),
            "tiles": info["tiles"], "blocks": info["blocks"],
            "max_err_over_gammaK_first64": round(worst, 4)}
     for p in probs:

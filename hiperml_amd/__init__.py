@@ -277,6 +277,27 @@ class GroupedPlan:
                                       ctypes.byref(by)))
         return {"tiles": nt.value, "blocks": nb.value, "device_bytes": by.value}
 
+    KINDS = ("wave2", "wave1", "shared")
+
+    def kinds(self):
+        """Kernel kind name per tile class (None for an empty class)."""
+        k = (ctypes.c_int32 * 3)()
+        check(lib.hmlSgemmGroupedKinds(self.ptr, k))
+        return [self.KINDS[x] if x >= 0 else None for x in k]
+
+    def set_kind(self, cls, kind):
+        check(lib.hmlSgemmGroupedSetKind(self.ptr, cls, self.KINDS.index(kind)))
+
+    def autotune(self, reps=5, stream=None):
+        """hmlSgemmGroupedAutotune: times every kernel kind per class and
+        keeps the fastest.  Returns {class: {kind: ms}} for the non-empty
+        classes and the chosen kinds."""
+        ms = (ctypes.c_float * 9)()
+        check(lib.hmlSgemmGroupedAutotune(self.ptr, reps, stream, ms))
+        times = {c: {k: ms[3 * c + i] for i, k in enumerate(self.KINDS)}
+                 for c in range(3) if ms[3 * c] >= 0}
+        return {"ms": times, "kinds": self.kinds()}
+
     def destroy(self):
         if self.ptr:
             lib.hmlSgemmGroupedDestroy(self.ptr)

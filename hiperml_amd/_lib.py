@@ -105,6 +105,10 @@ _SIGS = {
     "hmlSgemmGroupedRun": (c_i32, [c_void_p, c_i32, c_void_p]),
     "hmlSgemmGroupedInfo": (c_i32, [c_void_p, ctypes.POINTER(c_i64),
                                     ctypes.POINTER(c_i32), ctypes.POINTER(c_size)]),
+    "hmlSgemmGroupedKinds": (c_i32, [c_void_p, ctypes.POINTER(c_i32)]),
+    "hmlSgemmGroupedSetKind": (c_i32, [c_void_p, c_i32, c_i32]),
+    "hmlSgemmGroupedAutotune": (c_i32, [c_void_p, c_i32, c_void_p,
+                                        ctypes.POINTER(ctypes.c_float)]),
     "hmlSgemmGroupedDestroy": (c_i32, [c_void_p]),
     "hmlSgemvEx": (c_i32, [c_i32, c_i32, c_i64, c_i64, ctypes.c_float,
                            c_void_p, c_i64, c_void_p, c_i64, ctypes.c_float,

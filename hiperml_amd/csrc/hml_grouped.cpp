@@ -17,13 +17,14 @@
 
 struct hmlGroupedPlan_t {
   int32_t count = 0;
-  int64_t ntiles = 0;
-  int64_t cls_off[hml::kGroupClasses] = {};   // tile list of each class
-  int64_t cls_n[hml::kGroupClasses] = {};
+  std::vector<hml::GroupProblem> hprobs;  // host copy of the descriptors
+  std::vector<int> pcls;                  // tile class of each problem
+  int64_t ntiles = 0;                     // work-list entries, all classes
+  int64_t cls_off[hml::kGroupClasses] = {};  // region of each class in `tiles`
+  int64_t cls_cap[hml::kGroupClasses] = {};  // entries the region holds
+  int64_t cls_n[hml::kGroupClasses] = {};    // entries of the current kind
   int cls_blocks[hml::kGroupClasses] = {};
-  // per class: workgroups take blocks of 4 row tiles that share B through
-  // LDS (true) or each wave takes one row tile with its own copy of B
-  bool cls_shared[hml::kGroupClasses] = {};
+  int cls_kind[hml::kGroupClasses] = {};
   void *dev = nullptr;  // one allocation: problems | tiles
   hml::GroupProblem *probs = nullptr;
   hml::GroupTile *tiles = nullptr;
@@ -41,11 +42,91 @@ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 hmlStatus record(hmlGroupedPlan_t *p, hipStream_t st) {
   for (int c = 0; c < kGroupClasses; ++c) {
     if (!p->cls_n[c]) continue;
-    hipError_t e = launch_grouped(c, p->cls_shared[c], p->probs, p->tiles + p->cls_off[c],
+    hipError_t e = launch_grouped(c, p->cls_kind[c], p->probs, p->tiles + p->cls_off[c],
                                   p->cls_n[c], p->cls_blocks[c], st);
     if (e != hipSuccess) return from_hip(e, "grouped kernel");
   }
   return HML_STATUS_SUCCESS;
+}
+
+void drop_graph(hmlGroupedPlan_t *p) {
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->graph_stream) (void)hipStreamDestroy(p->graph_stream);
+  p->exec = nullptr;
+  p->graph = nullptr;
+  p->graph_stream = nullptr;
+}
+
+// Work list of class c for kernel kind `kind`: every (problem, row block)
+// entry, longest first (stable: problems of equal cost keep their order).
+// The cost of an entry is its A words per column pass times the passes,
+// plus its C words.
+void class_tiles(const hmlGroupedPlan_t *p, int c, int kind, std::vector<GroupTile> *out) {
+  int rt, ct;
+  grouped_kind_shape(c, kind, &rt, &ct);
+  const int64_t rows = grouped_unit_rows(c, kind);
+  std::vector<std::pair<int64_t, int32_t>> order;  // (cost, problem)
+  for (int32_t i = 0; i < p->count; ++i) {
+    if (p->pcls[i] != c) continue;
+    const GroupProblem &g = p->hprobs[i];
+    const int64_t passes = ceil_div(ceil_div(g.N, 16), ct);
+    order.emplace_back(passes * rows * g.K + rows * g.N * (1 + g.read_c) + 64, i);
+  }
+  std::stable_sort(order.begin(), order.end(),
+                   [](const std::pair<int64_t, int32_t> &a, const std::pair<int64_t, int32_t> &b) {
+                     return a.first > b.first;
+                   });
+  out->clear();
+  for (const auto &o : order) {
+    const int64_t nrb = ceil_div(p->hprobs[o.second].M, rows);
+    for (int64_t rb = 0; rb < nrb; ++rb) out->push_back(GroupTile{o.second, 0, rb});
+  }
+}
+
+// Switches class c to kernel kind `kind`: uploads its work list and sizes
+// its persistent grid.  Nothing may be running the plan.
+hmlStatus set_kind(hmlGroupedPlan_t *p, int c, int kind) {
+  std::vector<GroupTile> t;
+  class_tiles(p, c, kind, &t);
+  if ((int64_t)t.size() > p->cls_cap[c])
+    return fail(HML_STATUS_INVALID_VALUE, "grouped: work list overflow");
+  if (!t.empty()) {
+    hipError_t e = hipMemcpy(p->tiles + p->cls_off[c], t.data(), sizeof(GroupTile) * t.size(),
+                             hipMemcpyHostToDevice);
+    if (e != hipSuccess) return from_hip(e, "grouped work list upload");
+  }
+  p->ntiles += (int64_t)t.size() - p->cls_n[c];
+  p->cls_n[c] = (int64_t)t.size();
+  p->cls_kind[c] = kind;
+  // the workgroups the device holds at once, never more than the list
+  // needs (4 waves per workgroup, one entry per workgroup in the shared-B
+  // kind, one per wave otherwise)
+  int resident = grouped_class_blocks(c, kind);
+  if (resident <= 0) resident = 512;
+  const int64_t need = kind == kGroupShared ? p->cls_n[c] : ceil_div(p->cls_n[c], 4);
+  p->cls_blocks[c] = (int)std::max<int64_t>(1, std::min<int64_t>(resident, need));
+  drop_graph(p);  // a captured graph holds the old launches
+  return HML_STATUS_SUCCESS;
+}
+
+// Default kernel kind of each class.  Shared B measured faster for N <= 32
+// on the 512-problem suite (0.93 vs 0.97 ms and 1.55 vs 1.62 ms for its two
+// classes) and slower above (3.28 vs 2.96 ms: its 1 x 4 tiles hold half the
+// rows of the per-wave 2 x 4 kernel).  HML_GROUPED_KERNEL = wave, wave1 or
+// shared forces one kind for every class; hmlSgemmGroupedAutotune measures.
+void default_kinds(int kinds[kGroupClasses]) {
+  kinds[0] = kGroupShared;
+  kinds[1] = kGroupShared;
+  kinds[2] = kGroupWave2;
+  if (const char *k = getenv("HML_GROUPED_KERNEL")) {
+    int f = -1;
+    if (!strcmp(k, "wave")) f = kGroupWave2;
+    if (!strcmp(k, "wave1")) f = kGroupWave1;
+    if (!strcmp(k, "shared")) f = kGroupShared;
+    if (f >= 0)
+      for (int c = 0; c < kGroupClasses; ++c) kinds[c] = f;
+  }
 }
 
 }  // namespace
@@ -66,19 +147,8 @@ hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan_t **plan, int32_t count,
     return fail(HML_STATUS_INVALID_VALUE, "grouped: NULL argument or count <= 0");
   if (count > 65535) return fail(HML_STATUS_INVALID_VALUE, "grouped: at most 65535 problems");
   std::vector<GroupProblem> probs(count);
-  std::vector<int64_t> tile_cost[kGroupClasses];
-  std::vector<GroupTile> tiles[kGroupClasses];
-  // Kernel of each class.  Shared B measured faster for N <= 32 (0.93 vs
-  // 0.97 ms and 1.55 vs 1.62 ms on the 512-problem suite's classes) and
-  // slower above (3.28 vs 2.96 ms: its 1 x 4 tiles hold half the rows of
-  // the per-wave 2 x 4 kernel, which fits in registers only without the
-  // LDS staging).  HML_GROUPED_KERNEL=wave / =shared forces one kernel
-  // for every class.
-  bool cls_shared[kGroupClasses] = {true, true, false};
-  if (const char *kenv = getenv("HML_GROUPED_KERNEL")) {
-    if (!strcmp(kenv, "wave") || !strcmp(kenv, "shared"))
-      for (int c = 0; c < kGroupClasses; ++c) cls_shared[c] = kenv[0] == 's';
-  }
+  std::vector<int> pcls(count);
+  int64_t cap[kGroupClasses] = {};
   for (int32_t i = 0; i < count; ++i) {
     if (M[i] <= 0 || N[i] <= 0 || K[i] <= 0)
       return fail(HML_STATUS_INVALID_VALUE, "grouped: problem %d has a zero extent", i);
@@ -88,53 +158,33 @@ hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan_t **plan, int32_t count,
     if (N[i] > M[i])
       return fail(HML_STATUS_UNSUPPORTED,
                   "grouped: problem %d has N > M (only tall problems are grouped)", i);
-    const int cls = N[i] <= 16 ? 0 : (N[i] <= 32 ? 1 : 2);
-    int rt, ct;
-    grouped_class_shape(cls, &rt, &ct);
-    if (cls_shared[cls]) rt = grouped_shared_rt(cls);
-    // the tile body addresses A and B with 31-bit buffer offsets
-    if (((16.0 * rt - 1) * lda[i] + K[i] + 4) * 4.0 >= 2147483647.0 ||
-        ((K[i] + 64.0) * ldb[i] + 16.0 * ct) * 4.0 >= 2147483647.0)
+    // the tile bodies address A and B with 31-bit buffer offsets from a
+    // wave's first row (at most 2 row tiles of 16) and column pass (at most
+    // 4 column tiles of 16)
+    if ((31.0 * lda[i] + K[i] + 4) * 4.0 >= 2147483647.0 ||
+        ((K[i] + 64.0) * ldb[i] + 64.0) * 4.0 >= 2147483647.0)
       return fail(HML_STATUS_UNSUPPORTED, "grouped: problem %d strides too large", i);
     GroupProblem &p = probs[i];
     p.A = A[i]; p.lda = lda[i]; p.B = B[i]; p.ldb = ldb[i]; p.C = C[i]; p.ldc = ldc[i];
     p.M = M[i]; p.N = N[i]; p.K = K[i];
     p.read_c = beta != 0.0f ? 1 : 0;
     p.alpha = alpha; p.beta = beta;
-    const int64_t rows = 16 * rt * (cls_shared[cls] ? 4 : 1);  // rows of one work unit
-    const int64_t nrb = ceil_div(M[i], rows);
-    const int64_t passes = ceil_div(ceil_div(N[i], 16), ct);
-    // cost of one tile: A bytes per column pass plus C bytes, in 4-byte words
-    const int64_t cost = passes * rows * K[i] + rows * N[i] * (1 + p.read_c) + 64;
-    for (int64_t rb = 0; rb < nrb; ++rb) {
-      tiles[cls].push_back(GroupTile{i, 0, rb});
-      tile_cost[cls].push_back(cost);
-    }
+    pcls[i] = N[i] <= 16 ? 0 : (N[i] <= 32 ? 1 : 2);
+    cap[pcls[i]] += ceil_div(M[i], 16);  // the finest work list (16-row entries)
   }
   hmlGroupedPlan_t *pl = new hmlGroupedPlan_t();
   pl->count = count;
-  for (int c = 0; c < kGroupClasses; ++c) pl->cls_shared[c] = cls_shared[c];
-  std::vector<GroupTile> sorted;
+  pl->hprobs = probs;
+  pl->pcls = pcls;
+  int64_t total = 0;
   for (int c = 0; c < kGroupClasses; ++c) {
-    // longest-first order (stable: equal costs keep problem order)
-    std::vector<int64_t> order(tiles[c].size());
-    for (size_t t = 0; t < order.size(); ++t) order[t] = (int64_t)t;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int64_t a, int64_t b) { return tile_cost[c][a] > tile_cost[c][b]; });
-    pl->cls_off[c] = (int64_t)sorted.size();
-    pl->cls_n[c] = (int64_t)order.size();
-    for (int64_t t : order) sorted.push_back(tiles[c][t]);
-    // persistent grid: the workgroups the device holds at once (4 waves
-    // each), never more than needed to give every wave a tile
-    int resident = grouped_class_blocks(c, cls_shared[c]);
-    if (resident <= 0) resident = 512;
-    pl->cls_blocks[c] = (int)std::max<int64_t>(
-        1, std::min<int64_t>(resident, cls_shared[c] ? pl->cls_n[c] : ceil_div(pl->cls_n[c], 4)));
+    pl->cls_off[c] = total;
+    pl->cls_cap[c] = cap[c];
+    total += cap[c];
   }
-  pl->ntiles = (int64_t)sorted.size();
   const size_t o_probs = 0;
   const size_t o_tiles = align256(o_probs + sizeof(GroupProblem) * count);
-  pl->bytes = align256(o_tiles + sizeof(GroupTile) * std::max<size_t>(1, sorted.size()));
+  pl->bytes = align256(o_tiles + sizeof(GroupTile) * std::max<int64_t>(1, total));
   hipError_t e = hipMalloc(&pl->dev, pl->bytes);
   if (e != hipSuccess) {
     const size_t b = pl->bytes;
@@ -146,12 +196,14 @@ hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan_t **plan, int32_t count,
   pl->probs = (GroupProblem *)(base + o_probs);
   pl->tiles = (GroupTile *)(base + o_tiles);
   e = hipMemcpy(pl->probs, probs.data(), sizeof(GroupProblem) * count, hipMemcpyHostToDevice);
-  if (e == hipSuccess && !sorted.empty())
-    e = hipMemcpy(pl->tiles, sorted.data(), sizeof(GroupTile) * sorted.size(), hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
+  hmlStatus s = from_hip(e, "grouped plan upload");
+  int kinds[kGroupClasses];
+  default_kinds(kinds);
+  for (int c = 0; c < kGroupClasses && s == HML_STATUS_SUCCESS; ++c) s = set_kind(pl, c, kinds[c]);
+  if (s != HML_STATUS_SUCCESS) {
     (void)hipFree(pl->dev);
     delete pl;
-    return from_hip(e, "grouped plan upload");
+    return s;
   }
   *plan = pl;
   return HML_STATUS_SUCCESS;
@@ -213,12 +265,101 @@ hmlStatus hmlSgemmGroupedInfo(hmlGroupedPlan_t *plan, int64_t *ntiles,
   return HML_STATUS_SUCCESS;
 }
 
+hmlStatus hmlSgemmGroupedKinds(hmlGroupedPlan_t *plan, int32_t *kinds) {
+  if (!plan || !kinds) return fail(HML_STATUS_INVALID_VALUE, "plan or kinds is NULL");
+  for (int c = 0; c < kGroupClasses; ++c) kinds[c] = plan->cls_n[c] ? plan->cls_kind[c] : -1;
+  return HML_STATUS_SUCCESS;
+}
+
+hmlStatus hmlSgemmGroupedSetKind(hmlGroupedPlan_t *plan, int32_t cls, int32_t kind) {
+  clear_error();
+  if (!plan) return fail(HML_STATUS_INVALID_VALUE, "plan is NULL");
+  if (cls < 0 || cls >= kGroupClasses || kind < 0 || kind >= kGroupKinds)
+    return fail(HML_STATUS_INVALID_VALUE, "grouped: class %d / kind %d out of range", cls, kind);
+  hipError_t e = hipDeviceSynchronize();  // the old work list may still be in use
+  if (e != hipSuccess) return from_hip(e, "hipDeviceSynchronize");
+  return set_kind(plan, cls, kind);
+}
+
+hmlStatus hmlSgemmGroupedAutotune(hmlGroupedPlan_t *plan, int32_t reps, void *stream,
+                                  float *ms) {
+  clear_error();
+  if (!plan) return fail(HML_STATUS_INVALID_VALUE, "plan is NULL");
+  if (reps < 1 || reps > 1000) return fail(HML_STATUS_INVALID_VALUE, "reps must be 1..1000");
+  hipStream_t st = (hipStream_t)stream;
+  if (ms)
+    for (int i = 0; i < kGroupClasses * kGroupKinds; ++i) ms[i] = -1.0f;
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return from_hip(e, "hipDeviceSynchronize");
+  // the trial runs overwrite C: keep a packed copy and put it back
+  size_t cbytes = 0;
+  for (const GroupProblem &g : plan->hprobs) cbytes += (size_t)g.M * g.N * sizeof(float);
+  void *saved = nullptr;
+  if ((e = hipMalloc(&saved, cbytes)) != hipSuccess)
+    return fail(HML_STATUS_ALLOC_FAILED, "grouped autotune: copy of C (%zu bytes): %s", cbytes,
+                hipGetErrorString(e));
+  auto copy_c = [&](bool save) {
+    size_t off = 0;
+    hipError_t r = hipSuccess;
+    for (const GroupProblem &g : plan->hprobs) {
+      char *packed = (char *)saved + off;
+      const size_t row = (size_t)g.N * sizeof(float), pitch = (size_t)g.ldc * sizeof(float);
+      r = save ? hipMemcpy2DAsync(packed, row, g.C, pitch, row, g.M, hipMemcpyDeviceToDevice, st)
+               : hipMemcpy2DAsync(g.C, pitch, packed, row, row, g.M, hipMemcpyDeviceToDevice, st);
+      if (r != hipSuccess) break;
+      off += row * g.M;
+    }
+    return r != hipSuccess ? r : hipStreamSynchronize(st);
+  };
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hmlStatus s = from_hip(copy_c(true), "grouped autotune: save C");
+  const bool have_copy = s == HML_STATUS_SUCCESS;
+  if (s == HML_STATUS_SUCCESS) {
+    e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    s = from_hip(e, "hipEventCreate");
+  }
+  for (int c = 0; c < kGroupClasses && s == HML_STATUS_SUCCESS; ++c) {
+    if (!plan->cls_n[c]) continue;
+    int best = plan->cls_kind[c];
+    float best_ms = 0.0f;
+    for (int kind = 0; kind < kGroupKinds && s == HML_STATUS_SUCCESS; ++kind) {
+      if ((s = set_kind(plan, c, kind)) != HML_STATUS_SUCCESS) break;
+      auto run = [&] {
+        return launch_grouped(c, kind, plan->probs, plan->tiles + plan->cls_off[c],
+                              plan->cls_n[c], plan->cls_blocks[c], st);
+      };
+      e = run();  // warm-up
+      if (e == hipSuccess) e = hipEventRecord(e0, st);
+      for (int r = 0; r < reps && e == hipSuccess; ++r) e = run();
+      if (e == hipSuccess) e = hipEventRecord(e1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float t = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+      if ((s = from_hip(e, "grouped autotune run")) != HML_STATUS_SUCCESS) break;
+      t /= (float)reps;
+      if (ms) ms[c * kGroupKinds + kind] = t;
+      if (kind == 0 || t < best_ms) {
+        best = kind;
+        best_ms = t;
+      }
+    }
+    if (s == HML_STATUS_SUCCESS) s = set_kind(plan, c, best);
+  }
+  if (have_copy) {
+    const hipError_t er = copy_c(false);
+    if (s == HML_STATUS_SUCCESS) s = from_hip(er, "grouped autotune: restore C");
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(saved);
+  return s;
+}
+
 hmlStatus hmlSgemmGroupedDestroy(hmlGroupedPlan_t *plan) {
   if (!plan) return HML_STATUS_SUCCESS;
   (void)hipDeviceSynchronize();
-  if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
-  if (plan->graph) (void)hipGraphDestroy(plan->graph);
-  if (plan->graph_stream) (void)hipStreamDestroy(plan->graph_stream);
+  drop_graph(plan);
   if (plan->dev) (void)hipFree(plan->dev);
   delete plan;
   return HML_STATUS_SUCCESS;

@@ -191,19 +191,23 @@ struct GroupProblem {
 struct GroupTile {
   int32_t prob;
   int32_t pad;
-  int64_t rb;  // row block of 16 * rt rows (rt of the tile class)
+  int64_t rb;  // row block of grouped_unit_rows() rows
 };
-// Tile classes of the grouped kernel, by N: 0: N <= 16 (2 x 1 tiles of 16 x
-// 16), 1: N <= 32 (2 x 2), 2: larger N (1 x 4, several column passes).
+// Tile classes of the grouped kernel, by N: 0: N <= 16 (one column tile of
+// 16 per pass), 1: N <= 32 (two), 2: larger N (four, several passes).
 constexpr int kGroupClasses = 3;
-void grouped_class_shape(int cls, int *rt, int *ct);
-// Workgroups of the class kernel the device holds at once (0 if unknown);
-// shared_b: the kernel whose workgroups take blocks of 4 row tiles and
-// share B through LDS.
-int grouped_class_blocks(int cls, bool shared_b);
-// row tiles (of 16 rows) per wave of the shared-B kernel of a class
-int grouped_shared_rt(int cls);
-hipError_t launch_grouped(int cls, bool shared_b, const GroupProblem *probs,
+// Kernel kinds of a class: per-wave tiles of 2 or 1 row tiles of 16 rows
+// (k_grouped_dot), or workgroup blocks of 4 row tiles that share each stage
+// of B through LDS (k_grouped_dotb).
+constexpr int kGroupWave2 = 0, kGroupWave1 = 1, kGroupShared = 2;
+constexpr int kGroupKinds = 3;
+// row tiles of 16 rows per wave, and column tiles of 16 per pass
+void grouped_kind_shape(int cls, int kind, int *rt, int *ct);
+// rows of one work-list entry (GroupTile::rb counts these)
+int grouped_unit_rows(int cls, int kind);
+// Workgroups of the kernel the device holds at once (0 if unknown).
+int grouped_class_blocks(int cls, int kind);
+hipError_t launch_grouped(int cls, int kind, const GroupProblem *probs,
                           const GroupTile *tiles, int64_t ntiles, int nblocks, hipStream_t st);
 // k-means assignment (hml_k_kmeans.hip).  Screened form: kKmWaves waves
 // per workgroup, each on its own 16-row tile; NI = dims / 16 rounded up, NT

@@ -400,18 +400,19 @@ __global__ __launch_bounds__(64 * dot_max_waves(RT, CT)) void k_skinny_dot(DotPa
 
 // ---------------------------------------------------------------------------
 // Persistent grouped SGEMM: hundreds of independent skinny problems in one
-// graph launch.  The host sorts the problems into tile classes by N (2 x 1,
-// 2 x 2 or 2 x 4 tiles of 16-row x 16-column blocks), builds one descriptor
-// per problem and, per class, a list of (problem, row block) tiles sorted by
-// decreasing cost; one persistent kernel per class runs its list: wave w
-// takes tiles w, w + W, w + 2W, ... (W = waves launched), a static
-// longest-first schedule with no atomics or counters, so the launches can
-// be captured in a graph and replayed.  Each tile runs the DOT tile body
-// above (B read in place, whole K, no split) over its problem's column
-// tiles.  One kernel per class keeps each at its own register budget (a
-// single kernel dispatching on the class would pay for the largest); the
-// 4-column class stages A through LDS DMA, which is what lets its 2 x 4
-// tile fit 2 waves per SIMD without spilling.
+// graph launch.  The host sorts the problems into tile classes by N (1, 2
+// or 4 column tiles of 16 per pass), builds one descriptor per problem and,
+// per class, a list of (problem, row block) tiles sorted by decreasing
+// cost; one persistent kernel per class runs its list: wave w takes tiles
+// w, w + W, w + 2W, ... (W = waves launched), a static longest-first
+// schedule with no atomics or counters, so the launches can be captured in
+// a graph and replayed.  Each tile runs the DOT tile body above (B read in
+// place, whole K, no split) over its problem's column tiles, RT row tiles
+// of 16 per wave (2, or 1 for more resident waves).  One kernel per class
+// keeps each at its own register budget (a single kernel dispatching on
+// the class would pay for the largest); the 4-column class stages A through
+// LDS DMA, which is what lets its 2 x 4 tile fit 2 waves per SIMD without
+// spilling.
 // ---------------------------------------------------------------------------
 template <int RT, int CT>
 __global__ __launch_bounds__(256, 2) void k_grouped_dot(const GroupProblem *probs,
@@ -1253,60 +1254,70 @@ hipError_t launch_fill(float *dst, size_t n, float value, hipStream_t st) {
 
 namespace hml {
 
-int grouped_shared_rt(int cls);
-
 namespace {
-const void *grouped_fn(int cls) {
-  switch (cls) {
+// Kernel of (class, kind): index kind * kGroupClasses + class.  The
+// shared-B kind runs 1 x 4 tiles in the 4-column class (2 x 4 needs more
+// than the 256 registers of 2 waves per SIMD without the per-wave kernel's
+// LDS staging of A).
+constexpr int kGroupRt[kGroupKinds][kGroupClasses] = {{2, 2, 2}, {1, 1, 1}, {2, 2, 1}};
+constexpr int kGroupCt[kGroupClasses] = {1, 2, 4};
+
+const void *grouped_fn(int i) {
+  switch (i) {
     case 0: return (const void *)k_grouped_dot<2, 1>;
     case 1: return (const void *)k_grouped_dot<2, 2>;
     case 2: return (const void *)k_grouped_dot<2, 4>;
-    case kGroupClasses + 0: return (const void *)k_grouped_dotb<2, 1>;
-    case kGroupClasses + 1: return (const void *)k_grouped_dotb<2, 2>;
-    case kGroupClasses + 2: return (const void *)k_grouped_dotb<1, 4>;
+    case 3: return (const void *)k_grouped_dot<1, 1>;
+    case 4: return (const void *)k_grouped_dot<1, 2>;
+    case 5: return (const void *)k_grouped_dot<1, 4>;
+    case 6: return (const void *)k_grouped_dotb<2, 1>;
+    case 7: return (const void *)k_grouped_dotb<2, 2>;
+    case 8: return (const void *)k_grouped_dotb<1, 4>;
     default: return nullptr;
   }
 }
-size_t grouped_lds(int cls) {
-  static const int kCt[kGroupClasses] = {1, 2, 4};
-  return cls >= kGroupClasses
-             ? dotb_lds(grouped_shared_rt(cls - kGroupClasses), kCt[cls - kGroupClasses])
-             : 0;
+size_t grouped_lds(int cls, int kind) {
+  return kind == kGroupShared ? dotb_lds(kGroupRt[kind][cls], kGroupCt[cls]) : 0;
 }
 }  // namespace
 
-// Row tiles of the shared-B kernel of a class: the 4-column class runs 1 x 4
-// tiles there (2 x 4 needs more than the 256 registers of 2 waves per SIMD)
-int grouped_shared_rt(int cls) { return cls == 2 ? 1 : 2; }
-
-void grouped_class_shape(int cls, int *rt, int *ct) {
-  static const int kRt[kGroupClasses] = {2, 2, 2}, kCt[kGroupClasses] = {1, 2, 4};
-  *rt = kRt[cls];
-  *ct = kCt[cls];
+void grouped_kind_shape(int cls, int kind, int *rt, int *ct) {
+  *rt = kGroupRt[kind][cls];
+  *ct = kGroupCt[cls];
 }
 
-int grouped_class_blocks(int cls, bool shared_b) {
+int grouped_unit_rows(int cls, int kind) {
+  return 16 * kGroupRt[kind][cls] * (kind == kGroupShared ? kDotbWaves : 1);
+}
+
+int grouped_class_blocks(int cls, int kind) {
   static std::mutex mu;
-  static int cache[2 * kGroupClasses];
-  const int i = cls + (shared_b ? kGroupClasses : 0);
+  static int cache[kGroupKinds * kGroupClasses];
+  if (cls < 0 || cls >= kGroupClasses || kind < 0 || kind >= kGroupKinds) return 0;
+  const int i = kind * kGroupClasses + cls;
   std::lock_guard<std::mutex> lk(mu);
-  if (!cache[i]) cache[i] = resident_blocks(grouped_fn(i), grouped_lds(i));
+  if (!cache[i]) cache[i] = resident_blocks(grouped_fn(i), grouped_lds(cls, kind));
   return cache[i];
 }
 
-hipError_t launch_grouped(int cls, bool shared_b, const GroupProblem *probs,
+hipError_t launch_grouped(int cls, int kind, const GroupProblem *probs,
                           const GroupTile *tiles, int64_t ntiles, int nblocks, hipStream_t st) {
   if (ntiles <= 0) return hipSuccess;
+  if (cls < 0 || cls >= kGroupClasses || kind < 0 || kind >= kGroupKinds)
+    return hipErrorInvalidValue;
   if (nblocks < 1) nblocks = 1;
   const dim3 grid((unsigned)nblocks), blk(256);
-  const size_t lds = grouped_lds(cls + (shared_b ? kGroupClasses : 0));
-  switch (cls + (shared_b ? kGroupClasses : 0)) {
+  const size_t lds = grouped_lds(cls, kind);
+  switch (kind * kGroupClasses + cls) {
     case 0: hipLaunchKernelGGL((k_grouped_dot<2, 1>), grid, blk, 0, st, probs, tiles, ntiles); break;
     case 1: hipLaunchKernelGGL((k_grouped_dot<2, 2>), grid, blk, 0, st, probs, tiles, ntiles); break;
     case 2: hipLaunchKernelGGL((k_grouped_dot<2, 4>), grid, blk, 0, st, probs, tiles, ntiles); break;
-    case kGroupClasses + 0: hipLaunchKernelGGL((k_grouped_dotb<2, 1>), grid, blk, lds, st, probs, tiles, ntiles); break;
-    case kGroupClasses + 1: hipLaunchKernelGGL((k_grouped_dotb<2, 2>), grid, blk, lds, st, probs, tiles, ntiles); break;
-    case kGroupClasses + 2: hipLaunchKernelGGL((k_grouped_dotb<1, 4>), grid, blk, lds, st, probs, tiles, ntiles); break;
+    case 3: hipLaunchKernelGGL((k_grouped_dot<1, 1>), grid, blk, 0, st, probs, tiles, ntiles); break;
+    case 4: hipLaunchKernelGGL((k_grouped_dot<1, 2>), grid, blk, 0, st, probs, tiles, ntiles); break;
+    case 5: hipLaunchKernelGGL((k_grouped_dot<1, 4>), grid, blk, 0, st, probs, tiles, ntiles); break;
+    case 6: hipLaunchKernelGGL((k_grouped_dotb<2, 1>), grid, blk, lds, st, probs, tiles, ntiles); break;
+    case 7: hipLaunchKernelGGL((k_grouped_dotb<2, 2>), grid, blk, lds, st, probs, tiles, ntiles); break;
+    case 8: hipLaunchKernelGGL((k_grouped_dotb<1, 4>), grid, blk, lds, st, probs, tiles, ntiles); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

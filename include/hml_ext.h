@@ -218,6 +218,26 @@ hmlStatus hmlSgemmGroupedRun(hmlGroupedPlan plan, int32_t use_graph,
                              void *stream);
 hmlStatus hmlSgemmGroupedInfo(hmlGroupedPlan plan, int64_t *ntiles,
                               int32_t *nblocks, size_t *device_bytes);
+/* Kernel kind of each tile class (kinds[0..2], -1 for a class with no
+ * problems): HML_GROUPED_WAVE2 (each wave runs 32-row tiles),
+ * HML_GROUPED_WAVE1 (16-row tiles, more resident waves) or
+ * HML_GROUPED_SHARED (each workgroup runs a block of 4 row tiles that read
+ * every stage of B once, through LDS).  A new plan takes the measured
+ * default (shared for N <= 32, wave2 above) unless HML_GROUPED_KERNEL =
+ * wave, wave1 or shared names one kind for every class. */
+#define HML_GROUPED_WAVE2 0
+#define HML_GROUPED_WAVE1 1
+#define HML_GROUPED_SHARED 2
+hmlStatus hmlSgemmGroupedKinds(hmlGroupedPlan plan, int32_t *kinds);
+/* Sets the kernel kind of one class (waits for the device first). */
+hmlStatus hmlSgemmGroupedSetKind(hmlGroupedPlan plan, int32_t cls, int32_t kind);
+/* The plan-level autotuner: times every kernel kind of every non-empty
+ * class on the plan's own problems (one warm-up and `reps` timed runs on
+ * `stream`) and keeps the fastest per class.  ms, if not NULL, receives the
+ * mean milliseconds per run at ms[class * 3 + kind] (-1 where not run).  C
+ * is saved into a temporary device copy and restored, so it holds what it
+ * held before; the plan must not be running meanwhile. */
+hmlStatus hmlSgemmGroupedAutotune(hmlGroupedPlan plan, int32_t reps, void *stream, float *ms);
 hmlStatus hmlSgemmGroupedDestroy(hmlGroupedPlan plan);
 
 /* y = alpha * op(A) * x + beta * y; A is M x N in `order`; incx/incy must

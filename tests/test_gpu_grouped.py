@@ -32,13 +32,14 @@ def exact(A, B):
     return A.astype(np.int64) @ B.astype(np.int64)
 
 
-@pytest.mark.parametrize("kernel", ["default", "shared", "wave"])
+@pytest.mark.parametrize("kernel", ["default", "shared", "wave", "wave1"])
 def test_grouped_exact_direct(hml, kernel, monkeypatch):
-    """Every grouped kernel.  Shared-B workgroups take blocks of 4 row tiles
-    and share B through LDS (128 rows per block for N <= 32, 64 above);
-    per-wave kernels take one 32-row tile per wave.  The default uses the
-    shared-B kernel for N <= 32 and the per-wave kernel above;
-    HML_GROUPED_KERNEL=shared / =wave forces one kind for every class."""
+    """Every grouped kernel kind.  Shared-B workgroups take blocks of 4 row
+    tiles and share B through LDS (128 rows per block for N <= 32, 64
+    above); per-wave kernels take one 32-row (wave) or 16-row (wave1) tile
+    per wave.  The default uses the shared-B kernel for N <= 32 and the
+    32-row per-wave kernel above; HML_GROUPED_KERNEL forces one kind for
+    every class."""
     if kernel != "default":
         monkeypatch.setenv("HML_GROUPED_KERNEL", kernel)
     rng = np.random.default_rng(10)
@@ -47,6 +48,8 @@ def test_grouped_exact_direct(hml, kernel, monkeypatch):
     info = plan.info()
     if kernel == "wave":
         unit = lambda n: 32
+    elif kernel == "wave1":
+        unit = lambda n: 16
     elif kernel == "shared":
         unit = lambda n: 64 if n > 32 else 128
     else:
@@ -56,6 +59,54 @@ def test_grouped_exact_direct(hml, kernel, monkeypatch):
     hml.synchronize()
     for (M, N, K, dA, dB, dC), (A, B, C) in zip(probs, host):
         np.testing.assert_array_equal(dC.to_numpy((M, N)), exact(A, B).astype(np.float32))
+    plan.destroy()
+
+
+def test_grouped_set_kind_every_kernel(hml):
+    """All nine (class, kind) kernels through hmlSgemmGroupedSetKind, on
+    integer-valued problems of every class: exact results, and the graph
+    is re-captured after a kind changes."""
+    rng = np.random.default_rng(13)
+    probs, host = make_problems(hml, rng, 30, mmax=2500, kmax=600)
+    assert {0, 1, 2} <= {0 if p[1] <= 16 else (1 if p[1] <= 32 else 2) for p in probs}
+    plan = hml.GroupedPlan(probs, 1.0, 0.0)
+    for kind in hml.GroupedPlan.KINDS:
+        for c in range(3):
+            plan.set_kind(c, kind)
+        assert plan.kinds() == [kind] * 3
+        for graph in (False, True):
+            for M, N, K, dA, dB, dC in probs:
+                hml.check(hml.lib.hmlMemsetD8(dC.ptr, 0xFF, 4 * M * N))
+            plan.run(graph=graph)
+            hml.synchronize()
+            for (M, N, K, dA, dB, dC), (A, B, C) in zip(probs, host):
+                np.testing.assert_array_equal(dC.to_numpy((M, N)), exact(A, B).astype(np.float32),
+                                              err_msg="%s graph=%s" % (kind, graph))
+    plan.destroy()
+
+
+def test_grouped_autotune(hml):
+    """hmlSgemmGroupedAutotune times every kind of every class, keeps the
+    fastest, and leaves C as it was (beta = 1 here, so a trial run that
+    leaked into C would show); the tuned plan then computes C + A B
+    exactly."""
+    rng = np.random.default_rng(14)
+    probs, host = make_problems(hml, rng, 30, mmax=3000, kmax=600)
+    plan = hml.GroupedPlan(probs, 1.0, 1.0)
+    res = plan.autotune(reps=2)
+    print("grouped autotune:", res)
+    hml.synchronize()
+    for (M, N, K, dA, dB, dC), (A, B, C) in zip(probs, host):
+        np.testing.assert_array_equal(dC.to_numpy((M, N)), C)
+    assert sorted(res["ms"]) == [0, 1, 2]
+    for c, times in res["ms"].items():
+        assert all(t > 0 for t in times.values())
+        assert res["kinds"][c] == min(times, key=times.get)
+    plan.run(graph=True)
+    hml.synchronize()
+    for (M, N, K, dA, dB, dC), (A, B, C) in zip(probs, host):
+        want = C.astype(np.int64) + exact(A, B)
+        np.testing.assert_array_equal(dC.to_numpy((M, N)), want.astype(np.float32))
     plan.destroy()
 
 
