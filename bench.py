@@ -34,6 +34,18 @@ SWEEP_GEMV = [(16384, 16384, False), (16384, 16384, True),
 SWEEP_GEMM = [(8192, n, 8192) for n in (1, 2, 4, 8, 16, 32, 64, 128)]
 
 
+def dominant_kernel_label(hml, shp):
+    """The kernels of one SGEMV A x call of shape shp = (M, N, trans) under
+    the selected variant: the DOT streaming kernel, plus the split
+    reduction when the row split spans workgroups."""
+    v, _ = hml.sgemv_select(shp[0], shp[1], trans=shp[2])
+    fam, lpr, rb, split, wpb = v.astuple()
+    name = "k_gemv_dot_stream<%d,%d>" % (lpr, rb) if fam == 1 else "GEMV family %d" % fam
+    if split > wpb:
+        name += " + k_reduce_wide (row split %d over %d-wave workgroups)" % (split, wpb)
+    return "%s: one SGEMV A x call, %d x %d" % (name, shp[0], shp[1])
+
+
 def gemv_bytes(M, N):
     return 4.0 * (M * N + M + N)
 
@@ -230,8 +242,7 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": None,
-                "kernel": "k_gemv_dot_stream<64,2> + its split reduction: one SGEMV A x call, "
-                          "16384 x 16384",
+                "kernel": dominant_kernel_label(hml, d[2]),
                 "bytes_per_launch": d[4], "avg_launch_ms": round(d_ms, 5)}
     traffic_file = os.path.join(here, "profiles", "traffic_gemv_dot_16384.json")
     if os.path.exists(traffic_file):

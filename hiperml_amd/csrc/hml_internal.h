@@ -3,8 +3,10 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
-#include <stdint.h>
 #include <stddef.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
 
 namespace hml {
 
@@ -110,6 +112,25 @@ struct DotParams {
 };
 // kw = waves per workgroup (each with its own K chunk)
 bool skinny_dot_runnable(int rt, int ct, int kw);
+// Load kind of the skinny DOT kernels' A (row-major, leading dimension ld
+// floats).  Rows that do not start on 128-byte lines share a line between
+// the end of one 64-k stage and the start of the next; read
+// non-temporally, that line tends to leave L2 between the two stages and
+// is fetched twice (the grouped suite's odd-K problems fetched 1.35x their
+// bytes that way, 1.07x with cached loads; single problems with odd K ran
+// 6-14% faster cached).  Line-aligned rows share nothing and stream faster
+// non-temporally (8192 x 16 x 8192: 52 vs 58 us).  HML_DOT_A_LOADS = nt /
+// cached forces one kind.
+inline bool a_loads_cached(const void *A, int64_t ld) {
+  static const int force = [] {
+    const char *e = getenv("HML_DOT_A_LOADS");
+    if (e && !strcmp(e, "nt")) return 0;
+    if (e && !strcmp(e, "cached")) return 1;
+    return -1;
+  }();
+  if (force >= 0) return force == 1;
+  return ((uintptr_t)A % 128) != 0 || (ld * 4) % 128 != 0;
+}
 // Waves of the (rt, ct) tile kernel with 4-wave workgroups resident on the
 // current device at once (0 if unknown).
 int skinny_dot_resident_waves(int rt, int ct);

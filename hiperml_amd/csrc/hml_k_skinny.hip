@@ -141,7 +141,7 @@ __device__ __forceinline__ void dot_store(const DotParams &p, int64_t row,
 // of the stage issued last may still be in flight: the loads of the stage
 // being read and its B came earlier), and before a tile is refilled,
 // lgkmcnt(0) (its reads are done).
-template <int RT, int CT, bool BVEC, bool DMA>
+template <int RT, int CT, bool BVEC, bool DMA, int APOL = kBufNT>
 __device__ __forceinline__ void dot_tile_lds(const float *__restrict__ A, int64_t sAm,
                                              const float *__restrict__ B, int64_t sBk,
                                              int64_t sBn, int64_t M, int64_t N,
@@ -191,10 +191,10 @@ __device__ __forceinline__ void dot_tile_lds(const float *__restrict__ A, int64_
     if constexpr (DMA) {
 #pragma unroll
       for (int q = 0; q < NQ; ++q)
-        buf_load_lds16<kBufNT>(rs, T + j * TF + q * 256, aoff[q], so);
+        buf_load_lds16<APOL>(rs, T + j * TF + q * 256, aoff[q], so);
     } else {
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) a[j][q] = buf_load4<kBufNT>(rs, aoff[q], so);
+      for (int q = 0; q < NQ; ++q) a[j][q] = buf_load4<APOL>(rs, aoff[q], so);
     }
   };
   // k-step u (32 k) of stage s's B
@@ -345,7 +345,7 @@ __device__ __forceinline__ void dot_tile_lds(const float *__restrict__ A, int64_
   }
 }
 
-template <int RT, int CT, bool BVEC>
+template <int RT, int CT, bool BVEC, int APOL = kBufNT>
 __global__ __launch_bounds__(64 * dot_max_waves(RT, CT)) void k_skinny_dot(DotParams p) {
   constexpr int WF = dot_wave_floats(RT, CT);
   extern __shared__ f4 dot_lds[];
@@ -363,9 +363,9 @@ __global__ __launch_bounds__(64 * dot_max_waves(RT, CT)) void k_skinny_dot(DotPa
   int64_t ke = kb + p.kchunk;
   if (ke > p.K) ke = p.K;
   f4 acc[RT][CT];
-  dot_tile_lds<RT, CT, BVEC, dot_dma(RT, CT)>(p.A, p.sAm, p.B, p.sBk, p.sBn, p.M, p.N, row0,
-                                   ct0, kb, ke, reinterpret_cast<float *>(dot_lds) + w * WF,
-                                   acc);
+  dot_tile_lds<RT, CT, BVEC, dot_dma(RT, CT), APOL>(p.A, p.sAm, p.B, p.sBk, p.sBn, p.M, p.N,
+                                                   row0, ct0, kb, ke,
+                                                   reinterpret_cast<float *>(dot_lds) + w * WF, acc);
 
   if (kw == 1) {
 #pragma unroll
@@ -432,8 +432,8 @@ __global__ __launch_bounds__(256, 2) void k_grouped_dot(const GroupProblem *prob
     const int64_t nct = (gp.N + 15) / 16;
     for (int ct0 = 0; ct0 < nct; ct0 += CT) {
       f4 acc[RT][CT];
-      dot_tile_lds<RT, CT, false, dot_dma(RT, CT)>(gp.A, gp.lda, gp.B, gp.ldb, 1, gp.M, gp.N, row0,
-                                  ct0, 0, gp.K, T, acc);
+      dot_tile_lds<RT, CT, false, dot_dma(RT, CT), 0>(gp.A, gp.lda, gp.B, gp.ldb, 1, gp.M, gp.N,
+                                                     row0, ct0, 0, gp.K, T, acc);
 #pragma unroll
       for (int tt = 0; tt < RT; ++tt)
 #pragma unroll
@@ -769,7 +769,7 @@ __device__ __forceinline__ int dotb_off(int n, int kq) { return n * kDotStageK +
 // of the workgroup must call it with the same ct0, kb and ke (it holds
 // workgroup barriers); lds holds dotb_lds(RT, CT) bytes.  On return the
 // LDS may still be read by other waves: barrier before reusing it.
-template <int RT, int CT, bool BVEC>
+template <int RT, int CT, bool BVEC, int APOL = kBufNT>
 __device__ __forceinline__ void dotb_block(const float *__restrict__ A, int64_t sAm,
                                            const float *__restrict__ B, int64_t sBk,
                                            int64_t sBn, int64_t M, int64_t N, int64_t row0,
@@ -829,7 +829,7 @@ __device__ __forceinline__ void dotb_block(const float *__restrict__ A, int64_t 
     const __amdgpu_buffer_rsrc_t rs = s < nst ? ra : rnull;
     const int32_t so = (int32_t)(s * kDotStageK * 4);
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) a[j][q] = buf_load4<kBufNT>(rs, aoff[q], so);
+    for (int q = 0; q < NQ; ++q) a[j][q] = buf_load4<APOL>(rs, aoff[q], so);
   };
   auto loadB = [&](int j, int64_t s) {
     const __amdgpu_buffer_rsrc_t rs = s < nst ? rb : rnull;
@@ -999,8 +999,8 @@ __global__ __launch_bounds__(64 * kDotbWaves, 2) void k_grouped_dotb(const Group
     const int64_t nct = (gp.N + 15) / 16;
     for (int ct0 = 0; ct0 < nct; ct0 += CT) {
       f4 acc[RT][CT];
-      dotb_block<RT, CT, false>(gp.A, gp.lda, gp.B, gp.ldb, 1, gp.M, gp.N, row0, ct0, 0, gp.K,
-                                lds, acc);
+      dotb_block<RT, CT, false, 0>(gp.A, gp.lda, gp.B, gp.ldb, 1, gp.M, gp.N, row0, ct0, 0,
+                                   gp.K, lds, acc);
 #pragma unroll
       for (int tt = 0; tt < RT; ++tt)
 #pragma unroll
@@ -1023,7 +1023,7 @@ __global__ __launch_bounds__(64 * kDotbWaves, 2) void k_grouped_dotb(const Group
   }
 }
 
-template <int RT, int CT, bool BV>
+template <int RT, int CT, bool BV, int APOL>
 hipError_t run_dot(int kw, const DotParams &p, hipStream_t st) {
   const int64_t nrb = (p.M + 16 * RT - 1) / (16 * RT);
   const int64_t nct = (p.N + 15) / 16;
@@ -1032,36 +1032,40 @@ hipError_t run_dot(int kw, const DotParams &p, hipStream_t st) {
   if (nblk > 0x7fffffff || ncb > 65535) return hipErrorInvalidConfiguration;
   const size_t lds = (size_t)kw * dot_wave_floats(RT, CT) * sizeof(float);
   if (lds > 65536) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_skinny_dot<RT, CT, BV>),
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_skinny_dot<RT, CT, BV, APOL>),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((k_skinny_dot<RT, CT, BV>), dim3((unsigned)nblk, (unsigned)ncb),
+  hipLaunchKernelGGL((k_skinny_dot<RT, CT, BV, APOL>), dim3((unsigned)nblk, (unsigned)ncb),
                      dim3(64 * kw), lds, st, p);
   return hipGetLastError();
 }
 
 typedef hipError_t (*DotFn)(int, const DotParams &, hipStream_t);
 
+// cached: A read with ordinary (cached) loads instead of non-temporal ones
 template <int RT, int CT>
-DotFn dot_fn(bool bv) {
-  return bv ? run_dot<RT, CT, true> : run_dot<RT, CT, false>;
+DotFn dot_fn(bool bv, bool cached) {
+  return cached ? (bv ? run_dot<RT, CT, true, 0> : run_dot<RT, CT, false, 0>)
+                : (bv ? run_dot<RT, CT, true, kBufNT> : run_dot<RT, CT, false, kBufNT>);
 }
 
 // instantiated (rt, ct): ct <= 4, rt * ct <= 8
-DotFn find_dot(int rt, int ct, bool bv) {
+DotFn find_dot(int rt, int ct, bool bv, bool cached = false) {
   switch (rt * 16 + ct) {
-    case 17: return dot_fn<1, 1>(bv);
-    case 18: return dot_fn<1, 2>(bv);
-    case 20: return dot_fn<1, 4>(bv);
-    case 33: return dot_fn<2, 1>(bv);
-    case 34: return dot_fn<2, 2>(bv);
-    case 36: return dot_fn<2, 4>(bv);
-    case 65: return dot_fn<4, 1>(bv);
-    case 66: return dot_fn<4, 2>(bv);
+    case 17: return dot_fn<1, 1>(bv, cached);
+    case 18: return dot_fn<1, 2>(bv, cached);
+    case 20: return dot_fn<1, 4>(bv, cached);
+    case 33: return dot_fn<2, 1>(bv, cached);
+    case 34: return dot_fn<2, 2>(bv, cached);
+    case 36: return dot_fn<2, 4>(bv, cached);
+    case 65: return dot_fn<4, 1>(bv, cached);
+    case 66: return dot_fn<4, 2>(bv, cached);
     default: return nullptr;
   }
 }
+
+
 
 
 template <int RT, int CT, bool BV>
@@ -1189,7 +1193,7 @@ hipError_t launch_skinny_dot(int rt, int ct, bool bvec, int kw, const DotParams 
                              hipStream_t st) {
   if (!skinny_dot_runnable(rt, ct, kw) || p.kgroups < 1 || p.kchunk % kDotStageK != 0)
     return hipErrorInvalidValue;
-  DotFn fn = find_dot(rt, ct, bvec);
+  DotFn fn = find_dot(rt, ct, bvec, a_loads_cached(p.A, p.sAm));
   return fn(kw, p, st);
 }
 bool skinny_dotb_runnable(int rt, int ct) {
