@@ -278,10 +278,15 @@ class GroupedPlan:
         return {"tiles": nt.value, "blocks": nb.value, "device_bytes": by.value}
 
     KINDS = ("wave2", "wave1", "shared")
+    CLASSES = 4  # by N: 1..16, 17..32, 33..48, larger
+
+    @staticmethod
+    def class_of(N):
+        return (N + 15) // 16 - 1 if N <= 48 else 3
 
     def kinds(self):
         """Kernel kind name per tile class (None for an empty class)."""
-        k = (ctypes.c_int32 * 3)()
+        k = (ctypes.c_int32 * self.CLASSES)()
         check(lib.hmlSgemmGroupedKinds(self.ptr, k))
         return [self.KINDS[x] if x >= 0 else None for x in k]
 
@@ -292,10 +297,11 @@ class GroupedPlan:
         """hmlSgemmGroupedAutotune: times every kernel kind per class and
         keeps the fastest.  Returns {class: {kind: ms}} for the non-empty
         classes and the chosen kinds."""
-        ms = (ctypes.c_float * 9)()
+        nk = len(self.KINDS)
+        ms = (ctypes.c_float * (nk * self.CLASSES))()
         check(lib.hmlSgemmGroupedAutotune(self.ptr, reps, stream, ms))
-        times = {c: {k: ms[3 * c + i] for i, k in enumerate(self.KINDS)}
-                 for c in range(3) if ms[3 * c] >= 0}
+        times = {c: {k: ms[nk * c + i] for i, k in enumerate(self.KINDS)}
+                 for c in range(self.CLASSES) if ms[nk * c] >= 0}
         return {"ms": times, "kinds": self.kinds()}
 
     def destroy(self):

@@ -37,6 +37,11 @@ struct hmlGroupedPlan_t {
 namespace hml {
 namespace {
 
+static_assert(kGroupClasses == HML_GROUPED_CLASSES, "tile classes of the ABI");
+static_assert(kGroupWave2 == HML_GROUPED_WAVE2 && kGroupWave1 == HML_GROUPED_WAVE1 &&
+                  kGroupShared == HML_GROUPED_SHARED && kGroupKinds == 3,
+              "kernel kinds of the ABI");
+
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 hmlStatus record(hmlGroupedPlan_t *p, hipStream_t st) {
@@ -111,14 +116,16 @@ hmlStatus set_kind(hmlGroupedPlan_t *p, int c, int kind) {
 }
 
 // Default kernel kind of each class.  Shared B measured faster for N <= 32
-// on the 512-problem suite (0.93 vs 0.97 ms and 1.55 vs 1.62 ms for its two
-// classes) and slower above (3.28 vs 2.96 ms: its 1 x 4 tiles hold half the
-// rows of the per-wave 2 x 4 kernel).  HML_GROUPED_KERNEL = wave, wave1 or
-// shared forces one kind for every class; hmlSgemmGroupedAutotune measures.
+// on the 512-problem suite (0.84 vs 0.91 ms and 1.42 vs 1.48 ms for those
+// two classes) and slower above (3.11 vs 2.90 ms for N = 33..64 in one
+// class: its 1 x 4 tiles hold half the rows of the per-wave 2 x 4 kernel).
+// HML_GROUPED_KERNEL = wave, wave1 or shared forces one kind for every
+// class; hmlSgemmGroupedAutotune measures.
 void default_kinds(int kinds[kGroupClasses]) {
   kinds[0] = kGroupShared;
   kinds[1] = kGroupShared;
   kinds[2] = kGroupWave2;
+  kinds[3] = kGroupWave2;
   if (const char *k = getenv("HML_GROUPED_KERNEL")) {
     int f = -1;
     if (!strcmp(k, "wave")) f = kGroupWave2;
@@ -169,7 +176,7 @@ hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan_t **plan, int32_t count,
     p.M = M[i]; p.N = N[i]; p.K = K[i];
     p.read_c = beta != 0.0f ? 1 : 0;
     p.alpha = alpha; p.beta = beta;
-    pcls[i] = N[i] <= 16 ? 0 : (N[i] <= 32 ? 1 : 2);
+    pcls[i] = grouped_class_of(N[i]);
     cap[pcls[i]] += ceil_div(M[i], 16);  // the finest work list (16-row entries)
   }
   hmlGroupedPlan_t *pl = new hmlGroupedPlan_t();

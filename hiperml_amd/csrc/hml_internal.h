@@ -214,9 +214,11 @@ struct GroupTile {
   int32_t pad;
   int64_t rb;  // row block of grouped_unit_rows() rows
 };
-// Tile classes of the grouped kernel, by N: 0: N <= 16 (one column tile of
-// 16 per pass), 1: N <= 32 (two), 2: larger N (four, several passes).
-constexpr int kGroupClasses = 3;
+// Tile classes of the grouped kernel, by N: column tiles of 16 per pass,
+// 0: N <= 16 (one), 1: N <= 32 (two), 2: N <= 48 (three), 3: larger N
+// (four, several passes past 64).
+constexpr int kGroupClasses = 4;
+inline int grouped_class_of(int64_t N) { return N <= 48 ? (int)((N + 15) / 16) - 1 : 3; }
 // Kernel kinds of a class: per-wave tiles of 2 or 1 row tiles of 16 rows
 // (k_grouped_dot), or workgroup blocks of 4 row tiles that share each stage
 // of B through LDS (k_grouped_dotb).

@@ -92,7 +92,7 @@ __host__ __device__ constexpr int dot_tile_floats(int rt) { return rt * 16 * kDo
 // shapes: measured on 8192 x {64, 128} x 8192, 2-5% faster than the register
 // path; with fewer columns the extra LDS tile costs more occupancy than the
 // saved registers buy (up to 30% slower for N = 2).
-__host__ __device__ constexpr bool dot_dma(int, int ct) { return ct >= 4; }
+__host__ __device__ constexpr bool dot_dma(int, int ct) { return ct >= 3; }
 // floats of LDS per wave of k_skinny_dot: its A tile(s) (PS of them with
 // DMA), or the accumulators for the final sum
 __host__ __device__ constexpr int dot_wave_floats(int rt, int ct) {
@@ -1260,26 +1260,45 @@ namespace hml {
 
 namespace {
 // Kernel of (class, kind): index kind * kGroupClasses + class.  The
-// shared-B kind runs 1 x 4 tiles in the 4-column class (2 x 4 needs more
-// than the 256 registers of 2 waves per SIMD without the per-wave kernel's
-// LDS staging of A).
-constexpr int kGroupRt[kGroupKinds][kGroupClasses] = {{2, 2, 2}, {1, 1, 1}, {2, 2, 1}};
-constexpr int kGroupCt[kGroupClasses] = {1, 2, 4};
+// shared-B kind runs 1-row-tile tiles in the 3- and 4-column classes (2 x 3
+// and 2 x 4 need more than the 256 registers of 2 waves per SIMD without
+// the per-wave kernel's LDS staging of A).
+constexpr int kGroupRt[kGroupKinds][kGroupClasses] = {{2, 2, 2, 2}, {1, 1, 1, 1}, {2, 2, 1, 1}};
+constexpr int kGroupCt[kGroupClasses] = {1, 2, 3, 4};
 
-const void *grouped_fn(int i) {
-  switch (i) {
-    case 0: return (const void *)k_grouped_dot<2, 1>;
-    case 1: return (const void *)k_grouped_dot<2, 2>;
-    case 2: return (const void *)k_grouped_dot<2, 4>;
-    case 3: return (const void *)k_grouped_dot<1, 1>;
-    case 4: return (const void *)k_grouped_dot<1, 2>;
-    case 5: return (const void *)k_grouped_dot<1, 4>;
-    case 6: return (const void *)k_grouped_dotb<2, 1>;
-    case 7: return (const void *)k_grouped_dotb<2, 2>;
-    case 8: return (const void *)k_grouped_dotb<1, 4>;
-    default: return nullptr;
-  }
+template <int RT, int CT>
+hipError_t launch_gwave(dim3 grid, const GroupProblem *probs, const GroupTile *tiles,
+                        int64_t ntiles, hipStream_t st) {
+  hipLaunchKernelGGL((k_grouped_dot<RT, CT>), grid, dim3(256), 0, st, probs, tiles, ntiles);
+  return hipGetLastError();
 }
+template <int RT, int CT>
+hipError_t launch_gshared(dim3 grid, const GroupProblem *probs, const GroupTile *tiles,
+                          int64_t ntiles, hipStream_t st) {
+  hipLaunchKernelGGL((k_grouped_dotb<RT, CT>), grid, dim3(64 * kDotbWaves), dotb_lds(RT, CT), st,
+                     probs, tiles, ntiles);
+  return hipGetLastError();
+}
+typedef hipError_t (*GroupLaunch)(dim3, const GroupProblem *, const GroupTile *, int64_t,
+                                  hipStream_t);
+struct GroupKernel {
+  const void *fn;
+  GroupLaunch launch;
+};
+const GroupKernel kGroupKernels[kGroupKinds * kGroupClasses] = {
+    {(const void *)k_grouped_dot<2, 1>, launch_gwave<2, 1>},
+    {(const void *)k_grouped_dot<2, 2>, launch_gwave<2, 2>},
+    {(const void *)k_grouped_dot<2, 3>, launch_gwave<2, 3>},
+    {(const void *)k_grouped_dot<2, 4>, launch_gwave<2, 4>},
+    {(const void *)k_grouped_dot<1, 1>, launch_gwave<1, 1>},
+    {(const void *)k_grouped_dot<1, 2>, launch_gwave<1, 2>},
+    {(const void *)k_grouped_dot<1, 3>, launch_gwave<1, 3>},
+    {(const void *)k_grouped_dot<1, 4>, launch_gwave<1, 4>},
+    {(const void *)k_grouped_dotb<2, 1>, launch_gshared<2, 1>},
+    {(const void *)k_grouped_dotb<2, 2>, launch_gshared<2, 2>},
+    {(const void *)k_grouped_dotb<1, 3>, launch_gshared<1, 3>},
+    {(const void *)k_grouped_dotb<1, 4>, launch_gshared<1, 4>},
+};
 size_t grouped_lds(int cls, int kind) {
   return kind == kGroupShared ? dotb_lds(kGroupRt[kind][cls], kGroupCt[cls]) : 0;
 }
@@ -1300,7 +1319,7 @@ int grouped_class_blocks(int cls, int kind) {
   if (cls < 0 || cls >= kGroupClasses || kind < 0 || kind >= kGroupKinds) return 0;
   const int i = kind * kGroupClasses + cls;
   std::lock_guard<std::mutex> lk(mu);
-  if (!cache[i]) cache[i] = resident_blocks(grouped_fn(i), grouped_lds(cls, kind));
+  if (!cache[i]) cache[i] = resident_blocks(kGroupKernels[i].fn, grouped_lds(cls, kind));
   return cache[i];
 }
 
@@ -1310,21 +1329,8 @@ hipError_t launch_grouped(int cls, int kind, const GroupProblem *probs,
   if (cls < 0 || cls >= kGroupClasses || kind < 0 || kind >= kGroupKinds)
     return hipErrorInvalidValue;
   if (nblocks < 1) nblocks = 1;
-  const dim3 grid((unsigned)nblocks), blk(256);
-  const size_t lds = grouped_lds(cls, kind);
-  switch (kind * kGroupClasses + cls) {
-    case 0: hipLaunchKernelGGL((k_grouped_dot<2, 1>), grid, blk, 0, st, probs, tiles, ntiles); break;
-    case 1: hipLaunchKernelGGL((k_grouped_dot<2, 2>), grid, blk, 0, st, probs, tiles, ntiles); break;
-    case 2: hipLaunchKernelGGL((k_grouped_dot<2, 4>), grid, blk, 0, st, probs, tiles, ntiles); break;
-    case 3: hipLaunchKernelGGL((k_grouped_dot<1, 1>), grid, blk, 0, st, probs, tiles, ntiles); break;
-    case 4: hipLaunchKernelGGL((k_grouped_dot<1, 2>), grid, blk, 0, st, probs, tiles, ntiles); break;
-    case 5: hipLaunchKernelGGL((k_grouped_dot<1, 4>), grid, blk, 0, st, probs, tiles, ntiles); break;
-    case 6: hipLaunchKernelGGL((k_grouped_dotb<2, 1>), grid, blk, lds, st, probs, tiles, ntiles); break;
-    case 7: hipLaunchKernelGGL((k_grouped_dotb<2, 2>), grid, blk, lds, st, probs, tiles, ntiles); break;
-    case 8: hipLaunchKernelGGL((k_grouped_dotb<1, 4>), grid, blk, lds, st, probs, tiles, ntiles); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+  return kGroupKernels[kind * kGroupClasses + cls].launch(dim3((unsigned)nblocks), probs, tiles,
+                                                          ntiles, st);
 }
 
 }  // namespace hml

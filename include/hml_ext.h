@@ -199,8 +199,9 @@ size_t hmlSgemmDenseWorkspaceSize(int32_t precision, int64_t M, int64_t N,
 /* Grouped skinny SGEMM, no reference counterpart: C_i = alpha * A_i * B_i +
  * beta * C_i for `count` independent ROW-MAJOR problems (M_i >= N_i), given
  * as host arrays of device pointers and extents.  The plan owns the device
- * descriptors and, per tile class (N <= 16, N <= 32, larger), a longest-first
- * tile schedule for one persistent kernel.  Run launches one kernel per
+ * descriptors and, per tile class (N <= 16, <= 32, <= 48, larger: 1, 2, 3 or
+ * 4 column tiles of 16 per pass), a longest-first tile schedule for one
+ * persistent kernel.  Run launches one kernel per
  * non-empty class; with use_graph those launches are captured into a
  * hipGraph on the first run and replayed after.  The A/B/C pointers are
  * captured at creation: rerun after changing their contents, create a new
@@ -218,13 +219,14 @@ hmlStatus hmlSgemmGroupedRun(hmlGroupedPlan plan, int32_t use_graph,
                              void *stream);
 hmlStatus hmlSgemmGroupedInfo(hmlGroupedPlan plan, int64_t *ntiles,
                               int32_t *nblocks, size_t *device_bytes);
-/* Kernel kind of each tile class (kinds[0..2], -1 for a class with no
- * problems): HML_GROUPED_WAVE2 (each wave runs 32-row tiles),
- * HML_GROUPED_WAVE1 (16-row tiles, more resident waves) or
+/* Kernel kind of each of the HML_GROUPED_CLASSES tile classes (kinds[0..3],
+ * -1 for a class with no problems): HML_GROUPED_WAVE2 (each wave runs
+ * 32-row tiles), HML_GROUPED_WAVE1 (16-row tiles, more resident waves) or
  * HML_GROUPED_SHARED (each workgroup runs a block of 4 row tiles that read
  * every stage of B once, through LDS).  A new plan takes the measured
  * default (shared for N <= 32, wave2 above) unless HML_GROUPED_KERNEL =
  * wave, wave1 or shared names one kind for every class. */
+#define HML_GROUPED_CLASSES 4
 #define HML_GROUPED_WAVE2 0
 #define HML_GROUPED_WAVE1 1
 #define HML_GROUPED_SHARED 2
@@ -234,7 +236,8 @@ hmlStatus hmlSgemmGroupedSetKind(hmlGroupedPlan plan, int32_t cls, int32_t kind)
 /* The plan-level autotuner: times every kernel kind of every non-empty
  * class on the plan's own problems (one warm-up and `reps` timed runs on
  * `stream`) and keeps the fastest per class.  ms, if not NULL, receives the
- * mean milliseconds per run at ms[class * 3 + kind] (-1 where not run).  C
+ * mean milliseconds per run at ms[class * 3 + kind] for the 4 classes (-1
+ * where not run).  C
  * is saved into a temporary device copy and restored, so it holds what it
  * held before; the plan must not be running meanwhile. */
 hmlStatus hmlSgemmGroupedAutotune(hmlGroupedPlan plan, int32_t reps, void *stream, float *ms);

@@ -63,17 +63,17 @@ def test_grouped_exact_direct(hml, kernel, monkeypatch):
 
 
 def test_grouped_set_kind_every_kernel(hml):
-    """All nine (class, kind) kernels through hmlSgemmGroupedSetKind, on
+    """All twelve (class, kind) kernels through hmlSgemmGroupedSetKind, on
     integer-valued problems of every class: exact results, and the graph
     is re-captured after a kind changes."""
     rng = np.random.default_rng(13)
     probs, host = make_problems(hml, rng, 30, mmax=2500, kmax=600)
-    assert {0, 1, 2} <= {0 if p[1] <= 16 else (1 if p[1] <= 32 else 2) for p in probs}
+    assert {0, 1, 2, 3} <= {hml.GroupedPlan.class_of(p[1]) for p in probs}
     plan = hml.GroupedPlan(probs, 1.0, 0.0)
     for kind in hml.GroupedPlan.KINDS:
-        for c in range(3):
+        for c in range(hml.GroupedPlan.CLASSES):
             plan.set_kind(c, kind)
-        assert plan.kinds() == [kind] * 3
+        assert plan.kinds() == [kind] * hml.GroupedPlan.CLASSES
         for graph in (False, True):
             for M, N, K, dA, dB, dC in probs:
                 hml.check(hml.lib.hmlMemsetD8(dC.ptr, 0xFF, 4 * M * N))
@@ -98,7 +98,7 @@ def test_grouped_autotune(hml):
     hml.synchronize()
     for (M, N, K, dA, dB, dC), (A, B, C) in zip(probs, host):
         np.testing.assert_array_equal(dC.to_numpy((M, N)), C)
-    assert sorted(res["ms"]) == [0, 1, 2]
+    assert sorted(res["ms"]) == [0, 1, 2, 3]
     for c, times in res["ms"].items():
         assert all(t > 0 for t in times.values())
         assert res["kinds"][c] == min(times, key=times.get)
