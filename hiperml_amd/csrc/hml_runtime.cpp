@@ -258,6 +258,7 @@ struct HostStage {
   void *pinned[2] = {nullptr, nullptr};
   hipStream_t st = nullptr;
   hipEvent_t done[2] = {nullptr, nullptr};
+  bool unavailable = false;  // set-up failed once: plain copies from then on
 };
 std::vector<HostStage> g_stage;  // per device, guarded by g_compat_mutex
 
@@ -272,10 +273,13 @@ hmlStatus host_stage(HostStage **out) {
   if (e != hipSuccess) return from_hip(e, "hipGetDevice");
   if ((int)g_stage.size() <= dev) g_stage.resize(dev + 1);
   HostStage &h = g_stage[dev];
+  if (h.unavailable) return fail(HML_STATUS_ALLOC_FAILED, "pinned staging unavailable");
   if (!h.st) {
     hipStream_t st = nullptr;
-    if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess)
+    if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) {
+      h.unavailable = true;
       return from_hip(e, "staging stream");
+    }
     HostStage n;
     n.st = st;
     for (int i = 0; i < 2 && e == hipSuccess; ++i) {
@@ -288,6 +292,7 @@ hmlStatus host_stage(HostStage **out) {
         if (n.done[i]) (void)hipEventDestroy(n.done[i]);
       }
       (void)hipStreamDestroy(st);
+      h.unavailable = true;
       return fail(HML_STATUS_ALLOC_FAILED, "pinned staging buffers: %s", hipGetErrorString(e));
     }
     h = n;
@@ -298,11 +303,12 @@ hmlStatus host_stage(HostStage **out) {
 }  // namespace
 
 hmlStatus compat_h2d(void *dst, const void *src, size_t bytes) {
-  if (bytes < kStageMin || staging_off())
-    return from_hip(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
   HostStage *h = nullptr;
-  hmlStatus s = host_stage(&h);
-  if (s != HML_STATUS_SUCCESS) return s;
+  // no pinned memory (or staging switched off): one plain copy
+  if (bytes < kStageMin || staging_off() || host_stage(&h) != HML_STATUS_SUCCESS) {
+    clear_error();
+    return from_hip(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+  }
   hipError_t e = hipSuccess;
   size_t i = 0;
   for (size_t off = 0; off < bytes && e == hipSuccess; off += kStageChunk, ++i) {
@@ -319,11 +325,11 @@ hmlStatus compat_h2d(void *dst, const void *src, size_t bytes) {
 }
 
 hmlStatus compat_d2h(void *dst, const void *src, size_t bytes) {
-  if (bytes < kStageMin || staging_off())
-    return from_hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
   HostStage *h = nullptr;
-  hmlStatus s = host_stage(&h);
-  if (s != HML_STATUS_SUCCESS) return s;
+  if (bytes < kStageMin || staging_off() || host_stage(&h) != HML_STATUS_SUCCESS) {
+    clear_error();
+    return from_hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+  }
   const size_t nchunk = (bytes + kStageChunk - 1) / kStageChunk;
   auto issue = [&](size_t i) {
     const size_t off = i * kStageChunk, n = std::min(kStageChunk, bytes - off);

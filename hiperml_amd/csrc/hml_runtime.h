@@ -57,8 +57,8 @@ hmlStatus compat_buffers(size_t a_bytes, size_t b_bytes, size_t c_bytes,
 // Synchronous copies between a pageable host array and device memory for
 // the compat entries (hold compat_mutex()): large ones move in chunks
 // through two pinned staging buffers, host threads filling (draining) one
-// while the DMA engine drains (fills) the other.  HML_HOST_STAGING=off
-// makes them plain hipMemcpy calls.
+// while the DMA engine drains (fills) the other.  HML_HOST_STAGING=off, or
+// pinned memory that cannot be had, makes them plain hipMemcpy calls.
 hmlStatus compat_h2d(void *dst, const void *src, size_t bytes);
 hmlStatus compat_d2h(void *dst, const void *src, size_t bytes);
 
