@@ -10,9 +10,11 @@ namespace hml {
 constexpr int kDenseTileM = 128;  // FP32 kernel block tile
 constexpr int kDenseTileN = 128;
 constexpr int kDenseTileK = 32;   // padding granularity of K
-constexpr int kSplitTileM = 256;  // bf16 kernels' block tile
+constexpr int kSplitTileM = 256;  // 16-bit planes' row padding (largest block tile)
 constexpr int kSplitTileN = 128;
 constexpr int kSplitKBlock = 32;  // k-block of the bf16 plane layout
+// padded K up to which the 16-bit kernels use 128-row block tiles
+constexpr int64_t kDenseShortK = 1024;
 
 struct DenseParams {
   // S == 0: FP32 [Mp][Kp] / [Np][Kp] (B transposed, K contiguous);

@@ -30,7 +30,8 @@
 // layout, Ap[S][Kp/32][Mp][32] and Bp[S][Kp/32][Np][32] (B transposed): the
 // 32-wide k-step of a 256-row panel is then 16 KB of contiguous memory, read
 // with whole-line 16-byte loads and no bounds checks.  Block tile 256 x 128 x
-// 32, 8 waves in a 4 x 2 grid of 64 x 64 wave tiles (2 waves per SIMD), two
+// 32 (128 x 128 on short K), 8 (4) waves in a 4 x 2 (2 x 2) grid of 64 x 64
+// wave tiles (2 waves per SIMD), two
 // LDS buffers (one barrier per k-step), global loads of step t+1 in flight
 // while step t computes, LDS rows unpadded with the 16-byte chunk index
 // XOR-swizzled by the row (conflict-free row-contiguous writes and operand
@@ -50,8 +51,8 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 
-constexpr int BM = kDenseTileM, BN = kDenseTileN;   // FP32 kernel tile
-constexpr int BMH = kSplitTileM, BNH = kSplitTileN;  // bf16 kernels' tile
+constexpr int BM = kDenseTileM, BN = kDenseTileN;  // FP32 kernel tile
+constexpr int BNH = kSplitTileN;                    // 16-bit kernels' tile width
 constexpr int BKF = 16;        // k per step, FP32 kernel
 constexpr int PADF = 4;        // float elements of LDS row padding
 
@@ -294,44 +295,57 @@ __device__ __forceinline__ int sw_off(int x, int kq) {
 }
 
 // S = 2 / 3: bf16 planes, one accumulator; S = 2 with F16: scaled fp16
-// planes, h*h and the corrections in separate accumulators.
-template <int S, bool F16>
-__global__ __launch_bounds__(512) void k_dense_bf16(DenseParams p) {
-  __shared__ __attribute__((aligned(16))) uint16_t As[2][S][BMH * kSplitKBlock];
+// planes, h*h and the corrections in separate accumulators.  WM waves along
+// M: a block tile of 64 WM x 128 (WM = 4: 256 x 128, 8 waves, one workgroup
+// per CU by LDS; WM = 2: 128 x 128, 4 waves, two workgroups per CU, so one
+// workgroup's C stores run under the other's products on short K).
+template <int S, bool F16, int WM>
+__global__ __launch_bounds__(128 * WM, 2) void k_dense_bf16(DenseParams p) {
+  constexpr int TM = 64 * WM;       // block rows
+  constexpr int NT = 128 * WM;      // threads
+  constexpr int NA = TM * 4 / NT;   // 16-byte A chunks per thread and step
+  constexpr int NB = BNH * 4 / NT;  // 16-byte B chunks per thread and step
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][S][TM * kSplitKBlock];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[2][S][BNH * kSplitKBlock];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wr = wave >> 1, wc = wave & 1;  // 4 x 2 waves of 64 x 64
+  const int wr = wave >> 1, wc = wave & 1;  // WM x 2 waves of 64 x 64
   int64_t bm, bn;
-  block_coords(p.Mp / BMH, p.Np / BNH, &bm, &bn);
+  block_coords(p.Mp / TM, p.Np / BNH, &bm, &bn);
   const int64_t nsteps = p.Kp / kSplitKBlock;
-  const uint16_t *Ag = (const uint16_t *)p.Ap + bm * BMH * kSplitKBlock;
+  const uint16_t *Ag = (const uint16_t *)p.Ap + bm * TM * kSplitKBlock;
   const uint16_t *Bg = (const uint16_t *)p.Bp + bn * BNH * kSplitKBlock;
   const int64_t planeA = p.Mp * p.Kp, planeB = p.Np * p.Kp;
   const int64_t slabA = p.Mp * kSplitKBlock, slabB = p.Np * kSplitKBlock;
 
-  // Step t of the block: A rows bm*256 .. +255 (thread chunks c = tid and
-  // tid + 512: row c / 4, chunk c % 4) and B rows bn*128 .. +127 (c = tid),
-  // each a contiguous run of the k-blocked planes.
-  u4 ra[S][2], rb[S];
+  // Step t of the block: A rows bm*TM .. +TM-1 (thread chunks c = tid +
+  // NT i: row c / 4, chunk c % 4) and B rows bn*128 .. +127 (likewise), each
+  // a contiguous run of the k-blocked planes.
+  u4 ra[S][NA], rb[S][NB];
   auto gload = [&](int64_t t) {
 #pragma unroll
     for (int s = 0; s < S; ++s) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
-        ra[s][i] = *reinterpret_cast<const u4 *>(Ag + s * planeA + t * slabA + (tid + 512 * i) * 8);
-      rb[s] = *reinterpret_cast<const u4 *>(Bg + s * planeB + t * slabB + tid * 8);
+      for (int i = 0; i < NA; ++i)
+        ra[s][i] = *reinterpret_cast<const u4 *>(Ag + s * planeA + t * slabA + (tid + NT * i) * 8);
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+        rb[s][i] = *reinterpret_cast<const u4 *>(Bg + s * planeB + t * slabB + (tid + NT * i) * 8);
     }
   };
   auto lstore = [&](int buf) {
 #pragma unroll
     for (int s = 0; s < S; ++s) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const int c = tid + 512 * i;
+      for (int i = 0; i < NA; ++i) {
+        const int c = tid + NT * i;
         *reinterpret_cast<u4 *>(&As[buf][s][sw_off(c >> 2, c & 3)]) = ra[s][i];
       }
-      *reinterpret_cast<u4 *>(&Bs[buf][s][sw_off(tid >> 2, tid & 3)]) = rb[s];
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        const int c = tid + NT * i;
+        *reinterpret_cast<u4 *>(&Bs[buf][s][sw_off(c >> 2, c & 3)]) = rb[s][i];
+      }
     }
   };
 
@@ -414,14 +428,14 @@ __global__ __launch_bounds__(512) void k_dense_bf16(DenseParams p) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) ea[i][e] = split_exponent(p.rmaxA[bm * BMH + wr * 64 + i * 16 + 4 * g + e]);
+      for (int e = 0; e < 4; ++e) ea[i][e] = split_exponent(p.rmaxA[bm * TM + wr * 64 + i * 16 + 4 * g + e]);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int e = 0; e < 4; ++e)
-          store_c(p, bm * BMH + wr * 64 + i * 16 + 4 * g + e,
+          store_c(p, bm * TM + wr * 64 + i * 16 + 4 * g + e,
                   bn * BNH + wc * 64 + j * 16 + r16,
                   ldexpf(acc[i][j][e] + acc2[i][j][e], -(ea[i][e] + eb[j])));
   } else {
@@ -431,7 +445,7 @@ __global__ __launch_bounds__(512) void k_dense_bf16(DenseParams p) {
       for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int e = 0; e < 4; ++e)
-          store_c(p, bm * BMH + wr * 64 + i * 16 + 4 * g + e,
+          store_c(p, bm * TM + wr * 64 + i * 16 + 4 * g + e,
                   bn * BNH + wc * 64 + j * 16 + r16, acc[i][j][e]);
   }
 }
@@ -597,17 +611,40 @@ hipError_t launch_split_f16(const float *src, int64_t sR, int64_t sK, int64_t ro
                                     dst);
 }
 
+// Block height of the two-plane kernels (bf16x3, f16x3): 128-row tiles
+// (two workgroups per CU) when the k loop is short, 256-row tiles (more
+// reuse per load) otherwise; HML_DENSE_TILE_M = 128 / 256 forces one (for
+// measurements).  bf16x6's three planes leave LDS for one 128-row
+// workgroup per CU only, so it always runs 256-row tiles.
+int dense_tile_m(int S, int64_t Kp) {
+  static const int force = [] {
+    const char *e = getenv("HML_DENSE_TILE_M");
+    return e ? atoi(e) : 0;
+  }();
+  if (S == 3) return 256;
+  if (force == 128 || force == 256) return force;
+  return Kp <= kDenseShortK ? 128 : 256;
+}
+
 hipError_t launch_dense(int S, const DenseParams &p, hipStream_t st) {
   // S: 0 FP32, 2 bf16x3, 3 bf16x6, 4 scaled f16x3
-  const int64_t nblk = S == 0 ? (p.Mp / BM) * (p.Np / BN) : (p.Mp / BMH) * (p.Np / BNH);
+  const int tm = S == 0 ? BM : dense_tile_m(S, p.Kp);
+  const int64_t nblk = S == 0 ? (p.Mp / BM) * (p.Np / BN) : (p.Mp / tm) * (p.Np / BNH);
   if (nblk <= 0) return hipSuccess;
   if (nblk > 0x7fffffff) return hipErrorInvalidConfiguration;
   const dim3 grid((unsigned)nblk);
+  const bool t128 = tm == 128;
   switch (S) {
     case 0: hipLaunchKernelGGL(k_dense_f32, grid, dim3(256), 0, st, p); break;
-    case 2: hipLaunchKernelGGL((k_dense_bf16<2, false>), grid, dim3(512), 0, st, p); break;
-    case 3: hipLaunchKernelGGL((k_dense_bf16<3, false>), grid, dim3(512), 0, st, p); break;
-    case 4: hipLaunchKernelGGL((k_dense_bf16<2, true>), grid, dim3(512), 0, st, p); break;
+    case 2:
+      if (t128) hipLaunchKernelGGL((k_dense_bf16<2, false, 2>), grid, dim3(256), 0, st, p);
+      else hipLaunchKernelGGL((k_dense_bf16<2, false, 4>), grid, dim3(512), 0, st, p);
+      break;
+    case 3: hipLaunchKernelGGL((k_dense_bf16<3, false, 4>), grid, dim3(512), 0, st, p); break;
+    case 4:
+      if (t128) hipLaunchKernelGGL((k_dense_bf16<2, true, 2>), grid, dim3(256), 0, st, p);
+      else hipLaunchKernelGGL((k_dense_bf16<2, true, 4>), grid, dim3(512), 0, st, p);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
