@@ -55,8 +55,13 @@ def run_dense(hml, A, B, C0, alpha, beta, precision, order="row", ta=False, tb=F
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16x6", "f16x3"])
 @pytest.mark.parametrize("layout", LAYOUTS, ids=lambda l: "%s%s%s" % (l[0], "T" if l[1] else "N", "T" if l[2] else "N"))
 def test_small_integers_exact(hml, precision, layout):
+    """Integer-valued operands: every product and partial sum is exact, so
+    each precision must return the exact result.  The shapes run both block
+    heights of the two-plane kernels (128 rows up to a padded K of 1024,
+    256 rows above: K = 1500)."""
     rng = np.random.default_rng(1)
-    for (M, N, K) in ((128, 128, 32), (200, 300, 77), (5, 130, 1000), (257, 33, 129)):
+    for (M, N, K) in ((128, 128, 32), (200, 300, 77), (5, 130, 1000), (257, 33, 129),
+                      (300, 160, 1500)):
         A = int_matrix(rng, (M, K), -8, 8)
         B = int_matrix(rng, (K, N), -8, 8)
         C0 = int_matrix(rng, (M, N), -8, 8)
