@@ -67,15 +67,18 @@ def main():
                          "the learned selector, else the heuristic)")
     ap.add_argument("--quick", action="store_true",
                     help="only the shape sweep: leave out the reference-shape, dense "
-                         "split-precision and grouped suites of the JSON line")
+                         "split-precision, grouped, k-means and host-pointer suites of the "
+                         "JSON line")
     ap.add_argument("--dense", action="store_true",
                     help="also print the split-precision dense suite as a JSON line of its own")
     ap.add_argument("--grouped", action="store_true",
                     help="also print the grouped 512-problem suite as a JSON line of its own")
     ap.add_argument("--host", action="store_true",
-                    help="also time the host-pointer entry points (PCIe copies included)")
+                    help="with --quick or --only-extra: the host-pointer entry points "
+                         "(PCIe copies included) as a JSON line of their own")
     ap.add_argument("--kmeans", action="store_true",
-                    help="also run the k-means assignment suite (extra JSON line)")
+                    help="with --quick or --only-extra: the k-means assignment suite as a "
+                         "JSON line of its own")
     ap.add_argument("--only-extra", action="store_true",
                     help="skip the main sweep, run only the requested extra suites")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -105,9 +108,9 @@ def main():
         if args.grouped:
             print(json.dumps(bench_grouped(hml)))
         if args.kmeans:
-            bench_kmeans(hml, cpu=not args.no_cpu)
+            print(json.dumps(bench_kmeans(hml, cpu=not args.no_cpu)))
         if args.host:
-            bench_host(hml)
+            print(json.dumps(bench_host(hml)))
         return
     stream = hml.Stream()
     S = stream.ptr
@@ -279,6 +282,8 @@ def main():
         bufs.clear()
         extras["dense_split_precision"] = bench_dense(hml)
         extras["grouped_512"] = bench_grouped(hml)
+        extras["kmeans_assign"] = bench_kmeans(hml, cpu=not args.no_cpu)
+        extras["host_pointer_entries"] = bench_host(hml)
 
     out = {
         "metric": "SGEMM/SGEMV GFLOP/s + HBM GB/s vs plain FP32 at matched FP64-oracle error",
@@ -313,10 +318,10 @@ def main():
         print(json.dumps(bench_dense(hml)))
     if args.grouped and args.quick:
         print(json.dumps(bench_grouped(hml)))
-    if args.kmeans:
-        bench_kmeans(hml, cpu=not args.no_cpu)
-    if args.host:
-        bench_host(hml)
+    if args.kmeans and args.quick:
+        print(json.dumps(bench_kmeans(hml, cpu=not args.no_cpu)))
+    if args.host and args.quick:
+        print(json.dumps(bench_host(hml)))
 
 
 def bandwidth_probe(hml, src, S, nbytes=256 << 20, reps=5, read_src=None):
@@ -635,7 +640,7 @@ def bench_kmeans(hml, reps=5, rows=1 << 24, cpu=True):
             C.free()
         R.free()
         A.free()
-    print(json.dumps({"suite": "k-means assignment, %d rows" % rows, "results": out}))
+    return {"suite": "k-means assignment, %d rows" % rows, "results": out}
 
 
 def bench_host(hml, reps=3):
@@ -674,7 +679,7 @@ def bench_host(hml, reps=3):
             res[mode] = {"ms": round(dt * 1e3, 3), "GB_s": round(nbytes / dt / 1e9, 1)}
         os.environ.pop("HML_HOST_STAGING", None)
         out.append(res)
-    print(json.dumps({"suite": "host-pointer entry points, PCIe copies included", "results": out}))
+    return {"suite": "host-pointer entry points, PCIe copies included", "results": out}
 
 
 CALIB_BYTES = 256 << 20
