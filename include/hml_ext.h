@@ -169,7 +169,8 @@ size_t hmlSgemmWorkspaceSize(hmlOrder order, hmlOp transA, hmlOp transB,
                              const hmlSgemmVariant *variant);
 
 /* The variant hmlSgemmEx would run for these arguments, and which selector
- * produced it (0 built-in heuristic, 1 config table, 2 learned MLP). */
+ * produced it (0 built-in heuristic, 1 config table, 2 learned MLP; the
+ * order is given with hmlSelectorInstall). */
 hmlStatus hmlSgemmSelect(hmlOrder order, hmlOp transA, hmlOp transB,
                          int64_t M, int64_t N, int64_t K, float beta,
                          hmlSgemmVariant *variant, int32_t *source);
@@ -261,6 +262,8 @@ hmlStatus hmlSgemvExVariant(hmlOrder order, hmlOp trans, int64_t M,
 size_t hmlSgemvWorkspaceSize(hmlOrder order, hmlOp trans, int64_t M,
                              int64_t N, const hmlSgemvVariant *variant);
 
+/* The SGEMV variant hmlSgemvEx would run, and its source as for
+ * hmlSgemmSelect (the learned net chooses among its GEMV classes). */
 hmlStatus hmlSgemvSelect(hmlOrder order, hmlOp trans, int64_t M, int64_t N,
                          float beta, hmlSgemvVariant *variant,
                          int32_t *source);
@@ -371,8 +374,14 @@ hmlStatus hmlMlpLoad(hmlMlp *net, const char *path);
  * transB, beta==0 of the row-major equivalent problem) and its class space
  * (skinny MFMA variants by row tiles, column tiles, K split and waves per
  * block; dense FP32 tiles; GEMV kernels).  Mask: 1 for classes that can run
- * the shape.  Install: the net becomes the first selector of hmlSgemm /
- * hmlSgemmEx (decisions memoised per problem); NULL uninstalls. */
+ * the shape.  Selection order of hmlSgemm* / hmlSgemv*: a loaded table's
+ * rule for the problem's own shape bucket, the installed net, the nearest
+ * table rule, the built-in heuristic.  The net shipped beside the library
+ * (config/hml_selector_mi355x.bin) is installed on a process's first
+ * selection unless HML_SELECTOR is off / 0 (another value names a model
+ * file).  Install: the net replaces it (a host copy of its weights picks
+ * each first-seen problem on the calling thread, no device work; decisions
+ * memoised per problem, at most 4096); NULL uninstalls. */
 int32_t   hmlSelectorFeatureCount(void);
 int32_t   hmlSelectorClassCount(void);
 hmlStatus hmlSelectorFeatures(const hmlGemmShape *shape, float *features);
