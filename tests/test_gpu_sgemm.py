@@ -361,7 +361,10 @@ def test_check_kernel_matches_cpu_fp64(hml):
         np.testing.assert_allclose(st["abs_bound_fro"], np.linalg.norm(bnd), rtol=1e-12)
 
 
-FULL = [(8192, n, 8192) for n in (1, 2, 4, 8, 16, 32, 64, 128)] + [(2048, 16, 2048)]
+# the benchmark sizes, the reference app's shape, and odd-K shapes whose rows
+# miss 128-byte lines (the DOT kernel's cached-load path)
+FULL = ([(8192, n, 8192) for n in (1, 2, 4, 8, 16, 32, 64, 128)] + [(2048, 16, 2048)] +
+        [(8192, 16, 8191), (16384, 4, 3001), (65536, 32, 1000)])
 
 
 @pytest.mark.parametrize("shape", FULL, ids=lambda s: "x".join(map(str, s)))
