@@ -1,7 +1,9 @@
 // hml_grouped.cpp -- grouped (batched) skinny SGEMM: a plan holds the device
 // descriptors of many independent row-major problems and, per tile class
-// (by N), a longest-first tile schedule; running it launches one persistent
-// kernel per non-empty class, optionally replayed from a captured hipGraph.
+// (by N), a kernel kind and a balanced static work list; running it
+// launches one persistent kernel per non-empty class, optionally replayed
+// from a captured hipGraph.  hmlSgemmGroupedAutotune picks each class's
+// kind by timing all of them on the plan's own problems.
 // No reference counterpart (the reference runs one problem per call,
 // src/hml_sgemm.cu:120-211).
 #include <stdio.h>
