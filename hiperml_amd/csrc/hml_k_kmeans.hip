@@ -214,18 +214,21 @@ __global__ __launch_bounds__(64 * kKmWaves) void k_kmeans_assign(KmParams p) {
           }
         }
         // screen: threshold over the pass, then candidates
-        // s = |c|^2 - 2 x.c, m = tau ((|x| + |c|)^2 + |s|)
+        // s = |c|^2 - 2 x.c, m = tau ((|x| + |c|)^2 + |s|); each centroid's
+        // lower end s - m is kept from the threshold loop for the
+        // candidate test (the same float values as recomputing it)
         float T = __builtin_inff();
-        f4 nrm[NT], nrt[NT];
+        f4 lo[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-          nrm[t] = *reinterpret_cast<const f4 *>(cn + pb + 16 * t + 4 * q);
-          nrt[t] = *reinterpret_cast<const f4 *>(cr + pb + 16 * t + 4 * q);
+          const f4 nrm = *reinterpret_cast<const f4 *>(cn + pb + 16 * t + 4 * q);
+          const f4 nrt = *reinterpret_cast<const f4 *>(cr + pb + 16 * t + 4 * q);
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            const float s = __builtin_fmaf(-2.f, acc[t][i], nrm[t][i]);
-            const float sum = xr + nrt[t][i];
+            const float s = __builtin_fmaf(-2.f, acc[t][i], nrm[i]);
+            const float sum = xr + nrt[i];
             const float m = tau * __builtin_fmaf(sum, sum, __builtin_fabsf(s));
+            lo[t][i] = s - m;
             if (pb + 16 * t + 4 * q + i < kc) T = __builtin_fminf(T, s + m);
           }
         }
@@ -235,12 +238,8 @@ __global__ __launch_bounds__(64 * kKmWaves) void k_kmeans_assign(KmParams p) {
 #pragma unroll
         for (int t = 0; t < NT; ++t)
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const float s = __builtin_fmaf(-2.f, acc[t][i], nrm[t][i]);
-            const float sum = xr + nrt[t][i];
-            const float m = tau * __builtin_fmaf(sum, sum, __builtin_fabsf(s));
-            if (pb + 16 * t + 4 * q + i < kc && !(s - m > T)) cand |= 1u << (4 * t + i);
-          }
+          for (int i = 0; i < 4; ++i)
+            if (pb + 16 * t + 4 * q + i < kc && !(lo[t][i] > T)) cand |= 1u << (4 * t + i);
         // exact refinement, ascending k within the lane
         float bd = FLT_MAX;
         uint32_t bk = kNone;
