@@ -104,22 +104,30 @@ __device__ __forceinline__ float exact_dist(const float *xr, const float *cr, in
   return dist;
 }
 
+// 128 centroids per pass over up to 32 dims: at most 128 VGPRs (4 waves per
+// SIMD, two 8-wave workgroups per CU; without the bound the 32-dim tile took
+// 129 and ran one workgroup per CU: 32 x 256 on 2^24 rows 3.81 -> 3.14 ms).
+// The other tiles fit without it (bounding them measured slower), and the
+// 33..64-dim ones would spill.
 template <int NI, int NT, bool FUSED>
-__global__ __launch_bounds__(64 * kKmWaves) void k_kmeans_assign(KmParams p) {
+__global__ __launch_bounds__(64 * kKmWaves, (NT == 8 && NI <= 2) ? 4 : 1) void k_kmeans_assign(KmParams p) {
   constexpr int DP = 16 * NI + 4;  // LDS row stride (floats), 16-byte multiple
   extern __shared__ f4 km_lds4[];
   float *lds = reinterpret_cast<float *>(km_lds4);
   float *cs = lds;                   // KC x DP centroids of the current chunk
   float *cn = cs + p.KC * DP;        // their squared norms
-  float *cr = cn + p.KC;             // and norms
+  float *cmx = cn + p.KC;            // the largest norm of the chunk (4 floats)
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  float *xs = cr + p.KC + w * 16 * DP;  // this wave's 16 rows
+  float *xs = cmx + 4 + w * 16 * DP;  // this wave's 16 rows
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, r = lane & 15;
   const int D = p.D, K = p.K;
   const int nd4 = (D + 3) >> 2;
   const int nchunks = (K + p.KC - 1) / p.KC;
+  // margin per centroid: tau S_k (see the screen below); per row and chunk
+  // the bound tau2 (|x| + max |c|)^2 >= tau S_k is used for every centroid
   const float tau = 4.0f * (float)(D + 4) * 0x1p-24f;
+  const float tau2 = 2.0f * tau * (1.0f + 0x1p-10f);
 
   auto stage = [&](int c0) {
     __syncthreads();  // every wave is done with the previous chunk
@@ -132,7 +140,16 @@ __global__ __launch_bounds__(64 * kKmWaves) void k_kmeans_assign(KmParams p) {
       float s = 0.f;
       for (int d = 0; d < D; ++d) s = __builtin_fmaf(cs[k * DP + d], cs[k * DP + d], s);
       cn[k] = s;
-      cr[k] = __builtin_sqrtf(s);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      // largest squared norm (the zero padding adds nothing; a NaN norm is
+      // skipped: that centroid's screen is NaN, which always passes)
+      float mx = 0.f;
+      for (int k = threadIdx.x; k < p.KC; k += 64) mx = __builtin_fmaxf(mx, cn[k]);
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) mx = __builtin_fmaxf(mx, __shfl_xor(mx, off));
+      if (threadIdx.x == 0) cmx[0] = __builtin_sqrtf(mx);
     }
     __syncthreads();
   };
@@ -196,10 +213,13 @@ __global__ __launch_bounds__(64 * kKmWaves) void k_kmeans_assign(KmParams p) {
 
     float rbd = FLT_MAX;
     uint32_t rbk = kNone;
+    float thi = __builtin_inff();  // min of s_j + M over the centroids screened so far
     for (int c = 0; c < nchunks; ++c) {
       if (nchunks > 1) stage(c * p.KC);
       if (!live) continue;
       const int kc = min(p.KC, K - c * p.KC);
+      const float xc = xr + cmx[0];
+      const float M = tau2 * (xc * xc);
       for (int pb = 0; pb < kc; pb += 16 * NT) {
         f4 acc[NT];
 #pragma unroll
@@ -213,33 +233,56 @@ __global__ __launch_bounds__(64 * kKmWaves) void k_kmeans_assign(KmParams p) {
             for (int e = 0; e < 4; ++e) acc[t] = mfma4(cf[e], xv[i][e], acc[t]);
           }
         }
-        // screen: threshold over the pass, then candidates
-        // s = |c|^2 - 2 x.c, m = tau ((|x| + |c|)^2 + |s|); each centroid's
-        // lower end s - m is kept from the threshold loop for the
-        // candidate test (the same float values as recomputing it)
-        float T = __builtin_inff();
-        f4 lo[NT];
+        // Screen.  s_k = |c_k|^2 - 2 x.c_k.  The reference's float distance
+        // and |x|^2 + s_k are both within (2D+4) 2^-24 S_k of the exact
+        // distance, S_k = (|x| + |c_k|)^2 + |s_k|, so only centroids with
+        // s_k - m_k <= s_j + m_j for every j, m_k = tau S_k, can be the
+        // reference's argmin.  |s_k| <= (|x| + |c_k|)^2 gives
+        // m_k <= M = tau2 (|x| + max |c|)^2, hence the test
+        // s_k <= min_j (s_j + M) + M, with j over every centroid screened
+        // so far (earlier passes and chunks included).  A NaN s_k, or a NaN
+        // or infinite M, passes.
+        f4 sv[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
           const f4 nrm = *reinterpret_cast<const f4 *>(cn + pb + 16 * t + 4 * q);
-          const f4 nrt = *reinterpret_cast<const f4 *>(cr + pb + 16 * t + 4 * q);
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const float s = __builtin_fmaf(-2.f, acc[t][i], nrm[i]);
-            const float sum = xr + nrt[i];
-            const float m = tau * __builtin_fmaf(sum, sum, __builtin_fabsf(s));
-            lo[t][i] = s - m;
-            if (pb + 16 * t + 4 * q + i < kc) T = __builtin_fminf(T, s + m);
+          for (int i = 0; i < 4; ++i) sv[t][i] = __builtin_fmaf(-2.f, acc[t][i], nrm[i]);
+        }
+        // the lane's entries 4t + i, centroid pb + 16t + 4q + i, that lie in
+        // the chunk (every one but in a chunk's last, partial pass)
+        uint32_t valid = NT == 8 ? 0xffffffffu : (1u << (4 * NT)) - 1u;
+        if (pb + 16 * NT > kc) {  // wave-uniform
+          valid = 0;
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+            const int n = min(max(kc - pb - 16 * t - 4 * q, 0), 4);
+            valid |= ((1u << n) - 1u) << (4 * t);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              if (i >= n) sv[t][i] = __builtin_inff();  // out of the minimum
           }
         }
-        T = __builtin_fminf(T, __shfl_xor(T, 16));
-        T = __builtin_fminf(T, __shfl_xor(T, 32));
+        float smin = __builtin_inff();
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) smin = sv[t][i] < smin ? sv[t][i] : smin;
+        {
+          const float o16 = __shfl_xor(smin, 16);
+          smin = o16 < smin ? o16 : smin;
+          const float o32 = __shfl_xor(smin, 32);
+          smin = o32 < smin ? o32 : smin;
+        }
+        const float th = smin + M;
+        thi = th < thi ? th : thi;
+        const float lim = thi + M;
         uint32_t cand = 0;
 #pragma unroll
         for (int t = 0; t < NT; ++t)
 #pragma unroll
-          for (int i = 0; i < 4; ++i)
-            if (pb + 16 * t + 4 * q + i < kc && !(lo[t][i] > T)) cand |= 1u << (4 * t + i);
+          for (int i = 0; i < 4; ++i) cand |= sv[t][i] > lim ? 0u : 1u << (4 * t + i);
+        cand &= valid;
         // exact refinement, ascending k within the lane
         float bd = FLT_MAX;
         uint32_t bk = kNone;
@@ -455,15 +498,16 @@ KmeansPlan plan_kmeans(int64_t nrows, int D, int K, int mode) {
   pl.nt = K <= 16 ? 1 : K <= 32 ? 2 : K <= 64 ? 4 : 8;
   const int dp = 16 * pl.ni + 4;
   const int pass = 16 * pl.nt;
-  // LDS: centroid chunk (KC x dp) + squared norms and norms (2 KC) + 16 rows
-  // per wave, at most kKmLdsBytes.
+  // LDS: centroid chunk (KC x dp) + squared norms (KC) + the chunk's
+  // largest norm (4 floats, keeping 16-byte alignment) + 16 rows per wave,
+  // at most kKmLdsBytes.
   const int64_t rows_f = (int64_t)kKmWaves * 16 * dp;
-  const int64_t room = kKmLdsBytes / 4 - rows_f;
-  int64_t kc = room / (dp + 2) / pass * pass;
+  const int64_t room = kKmLdsBytes / 4 - rows_f - 4;
+  int64_t kc = room / (dp + 1) / pass * pass;
   const int64_t kpad = ((int64_t)K + pass - 1) / pass * pass;
   if (kc > kpad) kc = kpad;
   pl.KC = (int)kc;
-  pl.lds = (size_t)(kc * (dp + 2) + rows_f) * 4;
+  pl.lds = (size_t)(kc * (dp + 1) + 4 + rows_f) * 4;
   pl.tiles = (nrows + 15) / 16;
   return pl;
 }

@@ -141,3 +141,29 @@ def test_direct_kernel_dim_limit(hml):
     ctrds = np.zeros((3, 33), np.float32)
     with pytest.raises(hml.HmlError):
         run(hml, rows, ctrds, kernel="direct")
+
+
+def test_centroid_scales(hml):
+    """Centroids of very different norms (the screen's margin is bounded
+    with each chunk's largest norm), and an infinite and a NaN centroid
+    (never chosen; the infinite norm widens the screen to every centroid),
+    over several screening passes and LDS chunks."""
+    rng = np.random.default_rng(11)
+    for D, K in ((16, 200), (64, 300), (5, 40)):
+        print("case D=%d K=%d" % (D, K), flush=True)
+        for make in (clustered, near_ties):
+            rows, ctrds = make(rng, 2000, D, K)
+            ctrds = ctrds.copy()
+            ctrds[3] *= 1e4
+            ctrds[K // 2] *= 1e-3
+            want = oracle.kmeans_assign(rows, ctrds)
+            for kern in kernels(D):
+                np.testing.assert_array_equal(run(hml, rows, ctrds, kernel=kern), want,
+                                              err_msg="%s kernel=%s" % (make.__name__, kern))
+            ctrds[7, 0] = np.inf
+            ctrds[9] = np.nan
+            want = oracle.kmeans_assign(rows, ctrds)
+            assert not np.isin(want, [7, 9]).any()
+            for kern in kernels(D):
+                np.testing.assert_array_equal(run(hml, rows, ctrds, kernel=kern), want,
+                                              err_msg="inf/nan %s kernel=%s" % (make.__name__, kern))
