@@ -1,6 +1,6 @@
 // hml_grouped.cpp -- grouped (batched) skinny SGEMM: a plan holds the device
 // descriptors of many independent row-major problems and, per tile class
-// (by N), a kernel kind and a balanced static work list; running it
+// (by N), a kernel kind and a balanced static work list laid out by XCD; running it
 // launches one persistent kernel per non-empty class, optionally replayed
 // from a captured hipGraph.  hmlSgemmGroupedAutotune picks each class's
 // kind by timing all of them on the plan's own problems.
@@ -24,7 +24,8 @@ struct hmlGroupedPlan_t {
   int64_t ntiles = 0;                     // work-list entries, all classes
   int64_t cls_off[hml::kGroupClasses] = {};  // region of each class in `tiles`
   int64_t cls_cap[hml::kGroupClasses] = {};  // entries the region holds
-  int64_t cls_n[hml::kGroupClasses] = {};    // entries of the current kind
+  int64_t cls_n[hml::kGroupClasses] = {};    // list length of the current kind
+  int64_t cls_real[hml::kGroupClasses] = {};  // its entries (the rest are holes)
   int cls_blocks[hml::kGroupClasses] = {};
   int cls_kind[hml::kGroupClasses] = {};
   void *dev = nullptr;  // one allocation: problems | tiles
@@ -133,6 +134,56 @@ void balance_rounds(const std::vector<GroupTile> &in, const std::vector<int64_t>
   for (int64_t i = 0; i < n; ++i) (*out)[(i / U) * U + phys[unit[i]]] = in[i];
 }
 
+// Workgroups are dealt to the XCDs round robin: workgroup i runs on XCD
+// i % kXcds, each XCD with its own L2.
+constexpr int kXcds = 8;
+
+// balance_rounds with the units grouped by XCD, so that the entries one
+// XCD runs at a time come from few problems, whose B then stays in that
+// XCD's L2 (dealt over all units, a round's entries put every active
+// problem's B in every XCD's L2).  Round r's entries (longest first, each
+// problem's row blocks together) are cut into one contiguous slice per
+// XCD, sized by its units; the slices go, largest first, to the XCDs with
+// the least cost so far, and each slice's entries, largest first, to the
+// XCD's units with the least cost so far.  Unit u (a wave, u / per_block
+// its workgroup) runs positions u, u + U, ...; in the last, partial round
+// the units without an entry get holes (prob = -1, skipped by the
+// kernels).  Returns false (nothing laid out) with fewer workgroups than
+// XCDs or a list of one round, where the plain layout does as well.
+bool balance_rounds_xcd(const std::vector<GroupTile> &in, const std::vector<int64_t> &cost,
+                        int64_t U, int per_block, std::vector<GroupTile> *out) {
+  const int64_t n = (int64_t)in.size();
+  if (U / per_block < kXcds || n <= U) return false;
+  std::vector<std::vector<int64_t>> units(kXcds);
+  for (int64_t u = 0; u < U; ++u) units[(u / per_block) % kXcds].push_back(u);
+  std::vector<int64_t> load(U, 0), xload(kXcds, 0);
+  std::vector<int> xorder(kXcds);
+  const int64_t R = ceil_div(n, U);
+  out->assign((size_t)(R * U), GroupTile{-1, 0, 0});
+  int64_t last = 0;  // one past the last position written
+  for (int64_t r = 0; r < R; ++r) {
+    for (int x = 0; x < kXcds; ++x) xorder[x] = x;
+    std::stable_sort(xorder.begin(), xorder.end(),
+                     [&](int a, int b) { return xload[a] < xload[b]; });
+    int64_t i = r * U;
+    const int64_t e = std::min(n, i + U);
+    for (int k = 0; k < kXcds && i < e; ++k) {
+      std::vector<int64_t> &us = units[xorder[k]];
+      std::stable_sort(us.begin(), us.end(), [&](int64_t a, int64_t b) { return load[a] < load[b]; });
+      const int64_t m = std::min<int64_t>((int64_t)us.size(), e - i);
+      for (int64_t j = 0; j < m; ++j, ++i) {
+        const int64_t pos = r * U + us[j];
+        (*out)[pos] = in[i];
+        load[us[j]] += cost[i];
+        xload[xorder[k]] += cost[i];
+        last = std::max(last, pos + 1);
+      }
+    }
+  }
+  out->resize((size_t)last);
+  return true;
+}
+
 // Switches class c to kernel kind `kind`: uploads its work list and sizes
 // its persistent grid.  Nothing may be running the plan.
 hmlStatus set_kind(hmlGroupedPlan_t *p, int c, int kind) {
@@ -149,14 +200,19 @@ hmlStatus set_kind(hmlGroupedPlan_t *p, int c, int kind) {
   if (resident <= 0) resident = 512;
   const int64_t need = kind == kGroupShared ? n : ceil_div(n, 4);
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(resident, need));
-  balance_rounds(sorted, cost, kind == kGroupShared ? blocks : 4 * (int64_t)blocks, &t);
+  const int per_block = kind == kGroupShared ? 1 : 4;
+  const int64_t U = (int64_t)blocks * per_block;
+  if (!balance_rounds_xcd(sorted, cost, U, per_block, &t)) balance_rounds(sorted, cost, U, &t);
+  if ((int64_t)t.size() > p->cls_cap[c])
+    return fail(HML_STATUS_INVALID_VALUE, "grouped: work list overflow");
   if (!t.empty()) {
     hipError_t e = hipMemcpy(p->tiles + p->cls_off[c], t.data(), sizeof(GroupTile) * t.size(),
                              hipMemcpyHostToDevice);
     if (e != hipSuccess) return from_hip(e, "grouped work list upload");
   }
-  p->ntiles += n - p->cls_n[c];
-  p->cls_n[c] = n;
+  p->ntiles += n - p->cls_real[c];
+  p->cls_real[c] = n;
+  p->cls_n[c] = (int64_t)t.size();
   p->cls_kind[c] = kind;
   p->cls_blocks[c] = blocks;
   drop_graph(p);  // a captured graph holds the old launches
@@ -227,6 +283,10 @@ hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan_t **plan, int32_t count,
     pcls[i] = grouped_class_of(N[i]);
     cap[pcls[i]] += ceil_div(M[i], 16);  // the finest work list (16-row entries)
   }
+  // room for the holes of the XCD layout's last round (under one round of
+  // units: at most 4 waves per workgroup, 4096 workgroups)
+  for (int c = 0; c < kGroupClasses; ++c)
+    if (cap[c]) cap[c] += 16384;
   hmlGroupedPlan_t *pl = new hmlGroupedPlan_t();
   pl->count = count;
   pl->hprobs = probs;

@@ -427,6 +427,7 @@ __global__ __launch_bounds__(256, 2) void k_grouped_dot(const GroupProblem *prob
   float *T = reinterpret_cast<float *>(gtile[wv]);
   for (int64_t t = w; t < ntiles; t += W) {
     const GroupTile tile = tiles[t];
+    if (tile.prob < 0) continue;  // a hole of the list's layout
     const GroupProblem &gp = probs[tile.prob];
     const int64_t row0 = tile.rb * 16 * RT;
     const int64_t nct = (gp.N + 15) / 16;
@@ -994,6 +995,7 @@ __global__ __launch_bounds__(64 * kDotbWaves, 2) void k_grouped_dotb(const Group
   const int r = lane & 15, g = lane >> 4;
   for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const GroupTile tile = tiles[t];
+    if (tile.prob < 0) continue;  // a hole of the list's layout
     const GroupProblem &gp = probs[tile.prob];
     const int64_t row0 = (tile.rb * kDotbWaves + w) * 16 * RT;
     const int64_t nct = (gp.N + 15) / 16;
