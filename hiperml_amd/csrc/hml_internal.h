@@ -119,9 +119,11 @@ bool skinny_dot_runnable(int rt, int ct, int kw);
 // is fetched twice (the grouped suite's odd-K problems fetched 1.35x their
 // bytes that way, 1.07x with cached loads; single problems with odd K ran
 // 6-14% faster cached).  Line-aligned rows share nothing and stream faster
-// non-temporally (8192 x 16 x 8192: 52 vs 58 us).  HML_DOT_A_LOADS = nt /
-// cached forces one kind.
-inline bool a_loads_cached(const void *A, int64_t ld) {
+// non-temporally (8192 x 16 x 8192: 52 vs 58 us).  With more than one
+// column block (ncb), the blocks that share rows run on one XCD at about
+// the same time (xcd_block), and cached loads let the later ones read A
+// from that XCD's L2.  HML_DOT_A_LOADS = nt / cached forces one kind.
+inline bool a_loads_cached(const void *A, int64_t ld, int64_t ncb) {
   static const int force = [] {
     const char *e = getenv("HML_DOT_A_LOADS");
     if (e && !strcmp(e, "nt")) return 0;
@@ -129,7 +131,7 @@ inline bool a_loads_cached(const void *A, int64_t ld) {
     return -1;
   }();
   if (force >= 0) return force == 1;
-  return ((uintptr_t)A % 128) != 0 || (ld * 4) % 128 != 0;
+  return ((uintptr_t)A % 128) != 0 || (ld * 4) % 128 != 0 || ncb > 1;
 }
 // Waves of the (rt, ct) tile kernel with 4-wave workgroups resident on the
 // current device at once (0 if unknown).

@@ -112,6 +112,28 @@ __device__ __forceinline__ int dot_tile_off(int x, int kc) {
   return x * kDotStageK + 4 * (kc ^ (x & 15));
 }
 
+// Block coordinates of the skinny kernels' (row block x K group, column
+// block) grid.  Workgroups are dispatched in linear order L = y * gridDim.x
+// + x and dealt to the 8 XCDs round robin (XCD L % 8, each with its own
+// L2).  With several column blocks, each of which streams the same rows of
+// A, L is read so that the column blocks of one (row block, K group) land
+// on one XCD, dispatched 8 workgroups apart: the second reads of A come
+// from that XCD's L2 instead of HBM.  (A bijection whenever gridDim.x is a
+// multiple of 8; other grids keep the plain order.)
+__device__ __forceinline__ void xcd_block(int64_t *bx, int *by) {
+  const int64_t nx = gridDim.x;
+  const int ny = (int)gridDim.y;
+  if (ny == 1 || (nx & 7)) {
+    *bx = blockIdx.x;
+    *by = (int)blockIdx.y;
+    return;
+  }
+  const int64_t L = (int64_t)blockIdx.y * nx + blockIdx.x;
+  const int64_t j = L >> 3;
+  *by = (int)(j % ny);
+  *bx = (j / ny) * 8 + (L & 7);
+}
+
 __device__ __forceinline__ void dot_store(const DotParams &p, int64_t row,
                                           int64_t col, int kgi, float v) {
   if (row < p.M && col < p.N) {
@@ -355,10 +377,13 @@ __global__ __launch_bounds__(64 * dot_max_waves(RT, CT)) void k_skinny_dot(DotPa
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int kw = blockDim.x >> 6;
   const int r = lane & 15, g = lane >> 4;
-  const int64_t rb = blockIdx.x / p.kgroups;
-  const int kgi = (int)(blockIdx.x - rb * p.kgroups);
+  int64_t bx;
+  int by;
+  xcd_block(&bx, &by);
+  const int64_t rb = bx / p.kgroups;
+  const int kgi = (int)(bx - rb * p.kgroups);
   const int64_t row0 = rb * 16 * RT;
-  const int ct0 = blockIdx.y * CT;
+  const int ct0 = by * CT;
   const int64_t kb = ((int64_t)kgi * kw + w) * p.kchunk;
   int64_t ke = kb + p.kchunk;
   if (ke > p.K) ke = p.K;
@@ -511,10 +536,13 @@ __global__ __launch_bounds__(64 * axpy_max_waves(RT, CT)) void k_skinny_axpy(Ski
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int kw = blockDim.x >> 6;
   const int r = lane & 15, g = lane >> 4;
-  const int64_t rb = blockIdx.x / p.kgroups;
-  const int kgi = (int)(blockIdx.x - rb * p.kgroups);
+  int64_t bx;
+  int by;
+  xcd_block(&bx, &by);
+  const int64_t rb = bx / p.kgroups;
+  const int kgi = (int)(bx - rb * p.kgroups);
   const int64_t m0 = rb * kAxpyRowsPerTile * RT;
-  const int ct0 = blockIdx.y * CT;
+  const int ct0 = by * CT;
   const int64_t kb = ((int64_t)kgi * kw + w) * p.kchunk;
   int64_t ke = kb + p.kchunk;
   if (ke > p.K) ke = p.K;
@@ -957,10 +985,13 @@ __global__ __launch_bounds__(64 * kDotbWaves) void k_skinny_dotb(DotParams p) {
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int r = lane & 15, g = lane >> 4;
-  const int64_t rbk = blockIdx.x / p.kgroups;
-  const int kgi = (int)(blockIdx.x - rbk * p.kgroups);
+  int64_t bx;
+  int by;
+  xcd_block(&bx, &by);
+  const int64_t rbk = bx / p.kgroups;
+  const int kgi = (int)(bx - rbk * p.kgroups);
   const int64_t row0 = (rbk * kDotbWaves + w) * 16 * RT;
-  const int ct0 = blockIdx.y * CT;
+  const int ct0 = by * CT;
   const int64_t kb = (int64_t)kgi * p.kchunk;
   int64_t ke = kb + p.kchunk;
   if (ke > p.K) ke = p.K;
@@ -1195,7 +1226,8 @@ hipError_t launch_skinny_dot(int rt, int ct, bool bvec, int kw, const DotParams 
                              hipStream_t st) {
   if (!skinny_dot_runnable(rt, ct, kw) || p.kgroups < 1 || p.kchunk % kDotStageK != 0)
     return hipErrorInvalidValue;
-  DotFn fn = find_dot(rt, ct, bvec, a_loads_cached(p.A, p.sAm));
+  const int64_t ncb = ((p.N + 15) / 16 + ct - 1) / ct;
+  DotFn fn = find_dot(rt, ct, bvec, a_loads_cached(p.A, p.sAm, ncb));
   return fn(kw, p, st);
 }
 bool skinny_dotb_runnable(int rt, int ct) {
