@@ -193,14 +193,14 @@ hmlStatus set_kind(hmlGroupedPlan_t *p, int c, int kind) {
   if ((int64_t)sorted.size() > p->cls_cap[c])
     return fail(HML_STATUS_INVALID_VALUE, "grouped: work list overflow");
   // the workgroups the device holds at once, never more than the list
-  // needs (4 waves per workgroup, one entry per workgroup in the shared-B
-  // kind, one per wave otherwise)
+  // needs (one entry per workgroup in the shared-B kind, one per wave
+  // otherwise)
   const int64_t n = (int64_t)sorted.size();
   int resident = grouped_class_blocks(c, kind);
-  if (resident <= 0) resident = 512;
-  const int64_t need = kind == kGroupShared ? n : ceil_div(n, 4);
+  if (resident <= 0) resident = kind == kGroupShared ? 512 : 2048 / kGroupWaveWpb;
+  const int64_t need = kind == kGroupShared ? n : ceil_div(n, kGroupWaveWpb);
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(resident, need));
-  const int per_block = kind == kGroupShared ? 1 : 4;
+  const int per_block = kind == kGroupShared ? 1 : kGroupWaveWpb;
   const int64_t U = (int64_t)blocks * per_block;
   if (!balance_rounds_xcd(sorted, cost, U, per_block, &t)) balance_rounds(sorted, cost, U, &t);
   if ((int64_t)t.size() > p->cls_cap[c])
@@ -284,7 +284,7 @@ hmlStatus hmlSgemmGroupedCreate(hmlGroupedPlan_t **plan, int32_t count,
     cap[pcls[i]] += ceil_div(M[i], 16);  // the finest work list (16-row entries)
   }
   // room for the holes of the XCD layout's last round (under one round of
-  // units: at most 4 waves per workgroup, 4096 workgroups)
+  // units: at most 16 resident waves per CU)
   for (int c = 0; c < kGroupClasses; ++c)
     if (cap[c]) cap[c] += 16384;
   hmlGroupedPlan_t *pl = new hmlGroupedPlan_t();

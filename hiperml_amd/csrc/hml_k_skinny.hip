@@ -440,10 +440,10 @@ __global__ __launch_bounds__(64 * dot_max_waves(RT, CT)) void k_skinny_dot(DotPa
 // spilling.
 // ---------------------------------------------------------------------------
 template <int RT, int CT>
-__global__ __launch_bounds__(256, 2) void k_grouped_dot(const GroupProblem *probs,
+__global__ __launch_bounds__(64 * kGroupWaveWpb, 2) void k_grouped_dot(const GroupProblem *probs,
                                                      const GroupTile *tiles,
                                                      int64_t ntiles) {
-  __shared__ f4 gtile[4][(dot_dma(RT, CT) ? dot_stages(RT, CT) : 1) * dot_tile_floats(RT) / 4];
+  __shared__ f4 gtile[kGroupWaveWpb][(dot_dma(RT, CT) ? dot_stages(RT, CT) : 1) * dot_tile_floats(RT) / 4];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int r = lane & 15, g = lane >> 4;
@@ -1137,10 +1137,10 @@ DotbFn find_dotb(int rt, int ct, bool bv) {
 
 // Workgroups (256 threads, dynamic LDS `lds`) of kernel `fn` the current
 // device holds at once, from the occupancy calculator; 0 if unknown.
-int resident_blocks(const void *fn, size_t lds) {
+int resident_blocks(const void *fn, size_t lds, int threads = 256) {
   int nb = 0, dev = 0;
   hipDeviceProp_t prop;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, lds) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, threads, lds) != hipSuccess ||
       hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
     (void)hipGetLastError();
     return 0;
@@ -1303,7 +1303,8 @@ constexpr int kGroupCt[kGroupClasses] = {1, 2, 3, 4};
 template <int RT, int CT>
 hipError_t launch_gwave(dim3 grid, const GroupProblem *probs, const GroupTile *tiles,
                         int64_t ntiles, hipStream_t st) {
-  hipLaunchKernelGGL((k_grouped_dot<RT, CT>), grid, dim3(256), 0, st, probs, tiles, ntiles);
+  hipLaunchKernelGGL((k_grouped_dot<RT, CT>), grid, dim3(64 * kGroupWaveWpb), 0, st, probs, tiles,
+                     ntiles);
   return hipGetLastError();
 }
 template <int RT, int CT>
@@ -1353,7 +1354,9 @@ int grouped_class_blocks(int cls, int kind) {
   if (cls < 0 || cls >= kGroupClasses || kind < 0 || kind >= kGroupKinds) return 0;
   const int i = kind * kGroupClasses + cls;
   std::lock_guard<std::mutex> lk(mu);
-  if (!cache[i]) cache[i] = resident_blocks(kGroupKernels[i].fn, grouped_lds(cls, kind));
+  if (!cache[i])
+    cache[i] = resident_blocks(kGroupKernels[i].fn, grouped_lds(cls, kind),
+                               kind == kGroupShared ? 64 * kDotbWaves : 64 * kGroupWaveWpb);
   return cache[i];
 }
 
