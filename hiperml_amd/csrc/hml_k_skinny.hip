@@ -439,6 +439,37 @@ __global__ __launch_bounds__(64 * dot_max_waves(RT, CT)) void k_skinny_dot(DotPa
 // LDS DMA, which is what lets its 2 x 4 tile fit 2 waves per SIMD without
 // spilling.
 // ---------------------------------------------------------------------------
+// Stores a wave's RT x CT tile of one grouped problem, C = alpha acc
+// (+ beta C).  The descriptor's fields are read into registers once: as far
+// as the compiler knows the C stores may alias the descriptor array, so
+// fields read through the reference inside the loop were reloaded from
+// memory after every store, one dependent round trip per element.
+template <int RT, int CT>
+__device__ __forceinline__ void grouped_store(const GroupProblem &gp, int64_t row0, int ct0,
+                                              const f4 (&acc)[RT][CT]) {
+  float *const C = gp.C;
+  const int64_t ldc = gp.ldc, M = gp.M, N = gp.N;
+  const float alpha = gp.alpha, beta = gp.beta;
+  const bool read_c = gp.read_c != 0;
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int tt = 0; tt < RT; ++tt)
+#pragma unroll
+    for (int c = 0; c < CT; ++c)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t row = row0 + 16 * tt + 4 * g + i;
+        const int64_t col = (int64_t)(ct0 + c) * 16 + r;
+        if (row < M && col < N) {
+          float *cp = C + row * ldc + col;
+          float v = alpha * acc[tt][c][i];
+          if (read_c) v = alpha * acc[tt][c][i] + beta * (*cp);
+          *cp = v;
+        }
+      }
+}
+
 template <int RT, int CT>
 __global__ __launch_bounds__(64 * kGroupWaveWpb, 2) void k_grouped_dot(const GroupProblem *probs,
                                                      const GroupTile *tiles,
@@ -460,21 +491,7 @@ __global__ __launch_bounds__(64 * kGroupWaveWpb, 2) void k_grouped_dot(const Gro
       f4 acc[RT][CT];
       dot_tile_lds<RT, CT, false, dot_dma(RT, CT), 0>(gp.A, gp.lda, gp.B, gp.ldb, 1, gp.M, gp.N,
                                                      row0, ct0, 0, gp.K, T, acc);
-#pragma unroll
-      for (int tt = 0; tt < RT; ++tt)
-#pragma unroll
-        for (int c = 0; c < CT; ++c)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int64_t row = row0 + 16 * tt + 4 * g + i;
-            const int64_t col = (int64_t)(ct0 + c) * 16 + r;
-            if (row < gp.M && col < gp.N) {
-              float *cp = gp.C + row * gp.ldc + col;
-              float v = gp.alpha * acc[tt][c][i];
-              if (gp.read_c) v = gp.alpha * acc[tt][c][i] + gp.beta * (*cp);
-              *cp = v;
-            }
-          }
+      grouped_store<RT, CT>(gp, row0, ct0, acc);
     }
   }
 }
@@ -1034,21 +1051,7 @@ __global__ __launch_bounds__(64 * kDotbWaves, 2) void k_grouped_dotb(const Group
       f4 acc[RT][CT];
       dotb_block<RT, CT, false, 0>(gp.A, gp.lda, gp.B, gp.ldb, 1, gp.M, gp.N, row0, ct0, 0,
                                    gp.K, lds, acc);
-#pragma unroll
-      for (int tt = 0; tt < RT; ++tt)
-#pragma unroll
-        for (int c = 0; c < CT; ++c)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int64_t row = row0 + 16 * tt + 4 * g + i;
-            const int64_t col = (int64_t)(ct0 + c) * 16 + r;
-            if (row < gp.M && col < gp.N) {
-              float *cp = gp.C + row * gp.ldc + col;
-              float v = gp.alpha * acc[tt][c][i];
-              if (gp.read_c) v = gp.alpha * acc[tt][c][i] + gp.beta * (*cp);
-              *cp = v;
-            }
-          }
+      grouped_store<RT, CT>(gp, row0, ct0, acc);
       // the next block's first stage overwrites the LDS other waves may
       // still be reading
       __syncthreads();
