@@ -227,8 +227,8 @@ inline int grouped_class_of(int64_t N) { return N <= 48 ? (int)((N + 15) / 16) -
 constexpr int kGroupWave2 = 0, kGroupWave1 = 1, kGroupShared = 2;
 constexpr int kGroupKinds = 3;
 // waves per workgroup of the per-wave kinds (each wave its own entries).
-// One-wave workgroups measured 1-3% slower per class on the 512-problem
-// suite (the 4-column class 2.02 against 1.96 ms).
+// One-wave workgroups measured within 1% of four-wave ones per class on the
+// 512-problem suite (the 4-column class 1.74 against 1.75 ms).
 constexpr int kGroupWaveWpb = 4;
 // row tiles of 16 rows per wave, and column tiles of 16 per pass
 void grouped_kind_shape(int cls, int kind, int *rt, int *ct);
