@@ -170,7 +170,7 @@ def test_grouped_rejects_wide_problem(hml):
     assert e.value.status == 6
 
 
-def test_grouped_baseline_suite_vs_fp64(hml):
+def test_grouped_512_suite_vs_fp64(hml):
     """The benchmark's batched workload at full size: 512 problems, M and K
     log-uniform in [64, 65536] and [64, 4096], N uniform in 1..64 (fixed
     seed), uniform[-1,1] entries filled on the device.  Run directly and from
