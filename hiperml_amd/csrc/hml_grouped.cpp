@@ -202,9 +202,10 @@ hmlStatus set_kind(hmlGroupedPlan_t *p, int c, int kind) {
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(resident, need));
   const int per_block = kind == kGroupShared ? 1 : kGroupWaveWpb;
   const int64_t U = (int64_t)blocks * per_block;
-  if (!balance_rounds_xcd(sorted, cost, U, per_block, &t)) balance_rounds(sorted, cost, U, &t);
-  if ((int64_t)t.size() > p->cls_cap[c])
-    return fail(HML_STATUS_INVALID_VALUE, "grouped: work list overflow");
+  // the XCD layout when its holes fit the class's room, else the plain one
+  // (which has none)
+  if (!balance_rounds_xcd(sorted, cost, U, per_block, &t) || (int64_t)t.size() > p->cls_cap[c])
+    balance_rounds(sorted, cost, U, &t);
   if (!t.empty()) {
     hipError_t e = hipMemcpy(p->tiles + p->cls_off[c], t.data(), sizeof(GroupTile) * t.size(),
                              hipMemcpyHostToDevice);
