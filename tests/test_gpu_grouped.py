@@ -207,3 +207,35 @@ def test_grouped_512_suite_vs_fp64(hml):
                 np.testing.assert_array_equal(dC.to_numpy((M, N)), first[j])
         print("worst max_err/bound in units of gamma_K+2: %.3f" % worst)
     plan.destroy()
+
+
+def test_grouped_wide_and_short_problems(hml):
+    """Problems the suite's distribution leaves out: N past 64 (several
+    column passes of the 4-column class), and K below one 64-k stage down to
+    1, mixed with ordinary ones; every kind of every class, exact on
+    integer-valued data, with beta = 1 (C read back)."""
+    rng = np.random.default_rng(21)
+    shapes = [(300, 65, 1), (500, 96, 3), (2000, 130, 17), (777, 200, 64), (640, 70, 65),
+              (1500, 128, 300), (90, 1, 2), (400, 33, 5), (1000, 16, 63), (256, 24, 129)]
+    probs, host = [], []
+    for M, N, K in shapes:
+        A = int_matrix(rng, (M, K))
+        B = int_matrix(rng, (K, N))
+        C = int_matrix(rng, (M, N))
+        dA, dB, dC = (hml.DeviceBuffer.from_numpy(x) for x in (A, B, C))
+        probs.append((M, N, K, dA, dB, dC))
+        host.append((A, B, C))
+    plan = hml.GroupedPlan(probs, 1.0, 1.0)
+    for kind in hml.GroupedPlan.KINDS:
+        print("kind %s" % kind, flush=True)
+        for c in range(hml.GroupedPlan.CLASSES):
+            plan.set_kind(c, kind)
+        for (M, N, K, dA, dB, dC), (A, B, C) in zip(probs, host):
+            dC.upload(C)
+        plan.run(graph=False)
+        hml.synchronize()
+        for (M, N, K, dA, dB, dC), (A, B, C) in zip(probs, host):
+            want = (exact(A, B) + C.astype(np.int64)).astype(np.float32)
+            np.testing.assert_array_equal(dC.to_numpy((M, N)), want,
+                                          err_msg="kind %s M=%d N=%d K=%d" % (kind, M, N, K))
+    plan.destroy()
